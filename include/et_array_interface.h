@@ -1,0 +1,218 @@
+/*
+ * et_array_interface.h -- C ABI of lib_et_array_interface.so, the MI355X (gfx950)
+ * SPECT projector.
+ *
+ * Part 1 keeps NiftyRec's array interface so existing callers (the ctypes
+ * package Python/NiftyRec/NiftyRec/NiftyRec.py and the MATLAB mex files) load
+ * this library unchanged.  Each declaration cites the reference prototype it
+ * replaces (paths relative to the NiftyRec source tree).
+ *
+ * Conventions shared by every entry point:
+ *   - all arrays are float32 with x fastest: idx = x + nx*(y + ny*z);
+ *   - activity / attenuation [N, m, N], sinogram [N, m, n_cameras] (camera
+ *     slowest), psf [h, h, N] (one plane per depth), cameras [n_cameras x {1,3,6}]
+ *     stored cameras[axis*n_cameras + cam] in radians (one column: rotation
+ *     about axis 1); size arrays {0,0,0} mark an absent input;
+ *   - the caller owns every buffer; outputs are fully overwritten;
+ *   - return value 0 on success, otherwise a niftyrec status code (below);
+ *     the library never terminates the process.
+ *   - computation always runs on the HIP device selected by et_array_set_gpu
+ *     (default 0); the GPU / enable_gpu flags are accepted for compatibility.
+ *     Without a usable device the call returns niftyrec_error_nogpuinstalled.
+ *
+ * Part 2 adds device-pointer entry points (plans) for callers that keep data
+ * in GPU memory (the torch binding, the OSEM loop, the benchmark).
+ */
+#ifndef ET_ARRAY_INTERFACE_H
+#define ET_ARRAY_INTERFACE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes, reference emission-lib/_et_common.h:40-54. */
+#define niftyrec_success 0
+#define niftyrec_error_unspecified 1
+#define niftyrec_error_parameters 2
+#define niftyrec_error_kernel 3
+#define niftyrec_error_allocgpu 4
+#define niftyrec_error_alloccpu 5
+#define niftyrec_error_transfergpu 6
+#define niftyrec_error_nogpubuilt 7
+#define niftyrec_error_nogpuinstalled 8
+#define STATUS_SUCCESS 0
+#define STATUS_IO_ERROR 101
+#define STATUS_INITIALISATION_ERROR 103
+#define STATUS_PARAMETER_ERROR 104
+#define STATUS_UNHANDLED_ERROR 105
+#define STATUS_MEMORY_ERROR 106
+
+/* ---- Part 1: NiftyRec array interface ------------------------------------- */
+
+/* SimpleWrap status helpers, emission-lib/_et_array_interface.h:22-27. */
+int status_success(int *status);
+int status_io_error(int *status);
+int status_initialisation_error(int *status);
+int status_parameter_error(int *status);
+int status_unhandled_error(int *status);
+int echo(int *in, int *out);
+
+/* Message for a status code; thread-local buffer.
+ * Replaces niftyrec_error_msg, emission-lib/_et_common.h:56 (_et_common.cpp:269-306). */
+char *niftyrec_error_msg(int status);
+
+/* Device management, emission-lib/_et_array_interface.h:36-41
+ * (_et_array_interface.cpp:230-279).  info per device: {id, gflops, CUs, clock kHz, MB}. */
+int et_array_list_gpus(int *gpu_count, int *gpus_info_array);
+int et_array_set_gpu(int id);
+int et_array_set_gpu_pointer(int *id);
+int et_array_isinstalled(void);
+int et_array_reset_gpu(void);
+/* The kernels accept any volume size: both return values report that
+ * (the reference required multiples of 64, emission-lib/_et.cpp:27-36). */
+unsigned short int et_array_is_block_multiple(unsigned short int size);
+unsigned short int et_array_get_block_size(void);
+
+/* Projection, emission-lib/_et_array_interface.h:44 (_et_array_interface.cpp:337-541).
+ * Activity absent ({0,0,0}) -> background-only projection through the attenuation;
+ * as in the reference, activity_size is then overwritten with attenuation_size. */
+int SPECT_project_parallelholes(float *activity, int *activity_size, float *sinogram,
+                                int *sinogram_size, float *cameras, int *cameras_size,
+                                float *psf, int *psf_size, float *attenuation,
+                                int *attenuation_size, float *Background,
+                                float *Background_attenuation, int *Gpu,
+                                int *Truncate_negative_values);
+
+/* Backprojection (inverse-rotation pull, the reference operator),
+ * emission-lib/_et_array_interface.h:45 (_et_array_interface.cpp:547-710). */
+int SPECT_backproject_parallelholes(float *sino, int *sino_size, float *bkpr, int *bkpr_size,
+                                    float *cameras, int *cameras_size, float *psf,
+                                    int *psf_size, float *attenuation, int *attenuation_size,
+                                    float *Background, float *Background_attenuation, int *Gpu,
+                                    int *Truncate_negative_values);
+
+/* By-value spellings bound by the Python package,
+ * Python/NiftyRec/NiftyRec/NiftyRec.py:83-84 and :145-146. */
+int et_array_project(float *activity, int *activity_size, float *sinogram, int *sinogram_size,
+                     float *cameras, int *cameras_size, float *psf, int *psf_size,
+                     float *attenuation, int *attenuation_size, float background,
+                     float background_attenuation, int GPU, int truncate_negative_values);
+int et_array_backproject(float *sino, int *sino_size, float *bkpr, int *bkpr_size,
+                         float *cameras, int *cameras_size, float *psf, int *psf_size,
+                         float *attenuation, int *attenuation_size, float background,
+                         float background_attenuation, int GPU, int truncate_negative_values);
+
+/* Reference attenuation gradient, emission-lib/_et_array_interface.h:48
+ * (_et_array_interface.cpp:714-892; CUDA implementation emission-lib/_et.cpp:1679-1924). */
+int et_array_gradient_attenuation(float *sino, int *sino_size, float *activity,
+                                  int *activity_size, float *gradient, int *gradient_size,
+                                  float *cameras, int *cameras_size, float *psf, int *psf_size,
+                                  float *attenuation, int *attenuation_size, float background,
+                                  float background_attenuation, int GPU,
+                                  int truncate_negative_values);
+
+/* Standalone rotation resampler, emission-lib/_et_array_interface.h:32
+ * (_et_array_interface.cpp:112-156); axis_order 0..5 = XYZ,XZY,YXZ,YZX,ZXY,ZYX. */
+int et_array_rotate(float *image, int *size, float *rotated_image, float *angles,
+                    float *centers, float background, int axis_order, int GPU);
+/* Affine resampler, emission-lib/_et_array_interface.h:31 (_et_array_interface.cpp:56-108).
+ * affine is a row-major 4x4 matrix mapping output voxel to input voxel (the
+ * reference ignores it and resamples with a zero matrix, marked FIXME there). */
+int et_array_affine(float *image, int *image_size, float *transformed_image,
+                    int *transformed_image_size, float *affine, int *affine_size,
+                    float background, int GPU);
+/* Depth-dependent separable 2-D convolution of every z plane with the matching
+ * psf plane, emission-lib/_et_array_interface.h:33 (_et_array_interface.cpp:161-225). */
+int et_array_convolve(float *image, int *image_size, float *out, int *out_size, float *psf,
+                      int *psf_size, int GPU);
+
+/* PSF helpers kept with the reference's (stub) behaviour,
+ * emission-lib/_et_array_interface.h:53-54 (_et_array_interface.cpp:896-920). */
+int et_array_calculate_size_psf(unsigned int *psf_size_x, unsigned int *psf_size_y,
+                                float fwhm_pixels_dist0, float sensitivity0, float dist0,
+                                float fwhm_pixels_dist1, float sensitivity1, float dist1);
+int et_array_make_psf(float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y,
+                      float fwhm_pixels_dist0, float sensitivity0, float dist0,
+                      float fwhm_pixels_dist1, float sensitivity1, float dist1,
+                      unsigned int N_psf_planes);
+
+/* Cameras and EM drivers, emission-lib/_et_array_interface.h:55-59
+ * (_et_array_interface.cpp:923-1131). */
+int et_array_make_cameras(float *cameras_data, float firstcamera_deg, float lastcamera_deg,
+                          unsigned int n_cameras, unsigned int rotation_axis);
+int et_array_mlem_step(float *activity_data, int size_x, int size_y, float *sinogram_data,
+                       int n_cameras, float *cameras_data, int use_attenuation,
+                       float *attenuation_data, int use_psf, float *psf_data,
+                       unsigned int psf_size_x, unsigned int psf_size_y, float background,
+                       float background_attenuation, float epsilon, int use_gpu);
+int et_array_osem_step(float *activity_data, int size_x, int size_y, unsigned int subset_order,
+                       float *sinogram_data, int n_cameras, float *cameras_data,
+                       int use_attenuation, float *attenuation_data, int use_psf,
+                       float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y,
+                       float background, float background_attenuation, float epsilon,
+                       int use_gpu);
+int et_array_mlem(float *activity_data, unsigned int size_x, unsigned int size_y,
+                  float *sinogram_data, int n_cameras, float firstcamera, float lastcamera,
+                  unsigned int rotation_axis, int iterations, int use_attenuation,
+                  float *attenuation_data, int use_psf, float *psf_data,
+                  unsigned int psf_size_x, unsigned int psf_size_y, float background,
+                  float background_attenuation, float epsilon, int use_gpu);
+int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y,
+                  unsigned int subset_order, float *sinogram_data, int n_cameras,
+                  float firstcamera, float lastcamera, unsigned int rotation_axis,
+                  int iterations, int use_attenuation, float *attenuation_data, int use_psf,
+                  float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y,
+                  float background, float background_attenuation, float epsilon,
+                  int use_gpu);
+
+/* ---- Part 2: device-resident plans (no counterpart in the reference) ------- */
+/* A plan holds the geometry (cameras, centres, PSF, backgrounds) on the device
+ * and its own workspaces.  Array arguments below are DEVICE pointers in the
+ * layouts of Part 1; `stream` is a hipStream_t (NULL: the default stream).
+ * Calls on one plan must not overlap; distinct plans are independent. */
+typedef struct et_plan et_plan;
+
+/* cameras as in Part 1 (host memory, camera_cols 1, 3 or 6); psf host memory
+ * [psf_h, psf_h, nz] or NULL.  For camera_cols != 6 the rotation centres are
+ * ((nx-1)/2, (ny-1)/2, (nz-1)/2). */
+int et_plan_create(et_plan **plan, int nx, int ny, int nz, const float *cameras,
+                   int n_cameras, int camera_cols, const float *psf, int psf_h,
+                   float background, float background_attenuation);
+int et_plan_destroy(et_plan *plan);
+/* act or mu may be NULL (not both). */
+int et_plan_project(et_plan *plan, const float *act, const float *mu, float *sino,
+                    int truncate, void *stream);
+/* adjoint = 0: the reference backprojector; 1: exact transpose of et_plan_project. */
+int et_plan_backproject(et_plan *plan, const float *sino, const float *mu, float *out,
+                        int adjoint, int truncate, void *stream);
+/* exact = 0: the reference attenuation gradient; 1: gradient of <sino, project(act, mu)>
+ * with respect to mu (sino holds dL/dp). */
+int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *act,
+                                 const float *mu, float *out, int exact, int truncate,
+                                 void *stream);
+/* Ordered-subsets EM on the device: subset s holds views s, s+S, s+2S, ...;
+ * each subset update is the reference MLEM update restricted to its views.
+ * act is updated in place; the sensitivity of every subset is cached in the plan. */
+int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, int subsets,
+                 int iterations, float epsilon, void *stream);
+/* Tuning: depth chunks per ray (0 = automatic), views per backprojection batch (0 = automatic). */
+int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch);
+/* Kernel timing with HIP events on the plan's launches (category ids below). */
+int et_plan_profile(et_plan *plan, int enable);
+int et_plan_profile_read(et_plan *plan, int category, int *launches, double *total_ms,
+                         double *bytes);
+#define ET_PROF_PROJECT 0
+#define ET_PROF_BPU 1
+#define ET_PROF_PULL 2
+#define ET_PROF_MUSUM 3
+#define ET_PROF_OTHER 4
+#define ET_PROF_GRAD 5
+#define ET_PROF_COUNT 6
+
+/* Library version string. */
+const char *et_library_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

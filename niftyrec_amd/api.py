@@ -1,0 +1,271 @@
+"""numpy and torch front-end of the HIP SPECT projector.
+
+Conventions (numpy / torch C order, so shapes read reversed relative to the
+C sizes of the array interface):
+    volume    (nz, ny, nx)          C size [nx, ny, nz]
+    sinogram  (V, ny, nx)           C size [nx, ny, V]
+    psf       (nz, h, h)            C size [h, h, nz]
+    cameras   (V,) angles about the axial axis (y), or (3, V) per-axis angles,
+              or (6, V) angles followed by rotation centres; radians.
+
+numpy functions go through the NiftyRec array interface (host buffers,
+PCIe transfers included).  `Plan` keeps the geometry on the GPU and works on
+torch tensors already in device memory.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+REFERENCE = 0
+ADJOINT = 1
+
+
+def _f32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _cams(cameras):
+    """-> (float32 [axis][camera] memory, cols, V)."""
+    c = np.asarray(cameras, dtype=np.float32)
+    if c.ndim == 1:
+        return np.ascontiguousarray(c), 1, c.shape[0]
+    if c.shape[0] not in (1, 3, 6):
+        raise ValueError("cameras must be (V,), (3, V) or (6, V)")
+    return np.ascontiguousarray(c), c.shape[0], c.shape[1]
+
+
+def _size(a):
+    return (ctypes.c_int * 3)(*(a.shape[::-1] if a is not None else (0, 0, 0)))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_lib.fp)
+
+
+def project(activity, cameras, psf=None, attenuation=None, background=0.0, background_attenuation=0.0,
+            truncate_negative_values=True):
+    """Attenuated, PSF-blurred parallel-hole projection, (V, ny, nx)."""
+    L = _lib.load()
+    act, mu, psf = _f32(activity), _f32(attenuation), _f32(psf)
+    ref = act if act is not None else mu
+    cams, cols, V = _cams(cameras)
+    nz, ny, nx = ref.shape
+    sino = np.zeros((V, ny, nx), dtype=np.float32)
+    act_size = _size(act)
+    st = L.et_array_project(_p(act), act_size, _p(sino), (ctypes.c_int * 3)(nx, ny, V), _p(cams),
+                            (ctypes.c_int * 2)(V, cols), _p(psf), _size(psf), _p(mu), _size(mu),
+                            float(background), float(background_attenuation), 1, int(bool(truncate_negative_values)))
+    _lib.check(st, "project")
+    return sino
+
+
+def backproject(sinogram, cameras, psf=None, attenuation=None, background=0.0, background_attenuation=0.0,
+                truncate_negative_values=True, nz=None):
+    """Reference backprojection (inverse-rotation pull), volume (nz, ny, nx)."""
+    L = _lib.load()
+    sino, mu, psf = _f32(sinogram), _f32(attenuation), _f32(psf)
+    cams, cols, V = _cams(cameras)
+    _, ny, nx = sino.shape
+    nz = nx if nz is None else nz
+    out = np.zeros((nz, ny, nx), dtype=np.float32)
+    st = L.et_array_backproject(_p(sino), (ctypes.c_int * 3)(nx, ny, V), _p(out), (ctypes.c_int * 3)(nx, ny, nz),
+                                _p(cams), (ctypes.c_int * 2)(V, cols), _p(psf), _size(psf), _p(mu), _size(mu),
+                                float(background), float(background_attenuation), 1,
+                                int(bool(truncate_negative_values)))
+    _lib.check(st, "backproject")
+    return out
+
+
+def gradient_attenuation(sinogram, activity, attenuation, cameras, psf=None, background=0.0,
+                         background_attenuation=0.0, truncate_negative_values=True):
+    """The reference attenuation gradient (et_array_gradient_attenuation)."""
+    L = _lib.load()
+    sino, act, mu, psf = _f32(sinogram), _f32(activity), _f32(attenuation), _f32(psf)
+    cams, cols, V = _cams(cameras)
+    out = np.zeros_like(act)
+    nz, ny, nx = act.shape
+    st = L.et_array_gradient_attenuation(_p(sino), (ctypes.c_int * 3)(nx, ny, V), _p(act), _size(act), _p(out),
+                                         _size(out), _p(cams), (ctypes.c_int * 2)(V, cols), _p(psf), _size(psf),
+                                         _p(mu), _size(mu), float(background), float(background_attenuation), 1,
+                                         int(bool(truncate_negative_values)))
+    _lib.check(st, "gradient_attenuation")
+    return out
+
+
+def make_cameras(first_deg, last_deg, n, axis=1):
+    """(3, n) cameras with inclusive endpoints (et_array_make_cameras)."""
+    L = _lib.load()
+    out = np.zeros((3, n), dtype=np.float32)
+    _lib.check(L.et_array_make_cameras(_p(out), float(first_deg), float(last_deg), int(n), int(axis)),
+               "make_cameras")
+    return out
+
+
+def _cams3(cameras):
+    cams, cols, V = _cams(cameras)
+    if cols == 1:
+        c3 = np.zeros((3, V), dtype=np.float32)
+        c3[1] = cams
+        return c3, V
+    if cols != 3:
+        raise ValueError("EM steps take (V,) or (3, V) cameras")
+    return cams, V
+
+
+def mlem_step(activity, sinogram, cameras, psf=None, attenuation=None, background=0.0, background_attenuation=0.0,
+              epsilon=1e-6):
+    """One reference MLEM update (et_array_mlem_step); returns the new activity."""
+    L = _lib.load()
+    act = np.array(activity, dtype=np.float32, copy=True, order="C")
+    sino, mu, psf = _f32(sinogram), _f32(attenuation), _f32(psf)
+    c3, V = _cams3(cameras)
+    nz, ny, nx = act.shape
+    h = psf.shape[-1] if psf is not None else 0
+    st = L.et_array_mlem_step(_p(act), nx, ny, _p(sino), V, _p(c3), int(mu is not None), _p(mu),
+                              int(psf is not None), _p(psf), h, h, float(background), float(background_attenuation),
+                              float(epsilon), 1)
+    _lib.check(st, "mlem_step")
+    return act
+
+
+def osem_step(activity, sinogram, cameras, subset_order, psf=None, attenuation=None, background=0.0,
+              background_attenuation=0.0, epsilon=1e-6):
+    """One reference OSEM update on a random camera subset drawn with glibc
+    rand() (et_array_osem_step); returns the new activity."""
+    L = _lib.load()
+    act = np.array(activity, dtype=np.float32, copy=True, order="C")
+    sino, mu, psf = _f32(sinogram), _f32(attenuation), _f32(psf)
+    c3, V = _cams3(cameras)
+    nz, ny, nx = act.shape
+    h = psf.shape[-1] if psf is not None else 0
+    st = L.et_array_osem_step(_p(act), nx, ny, int(subset_order), _p(sino), V, _p(c3), int(mu is not None), _p(mu),
+                              int(psf is not None), _p(psf), h, h, float(background), float(background_attenuation),
+                              float(epsilon), 1)
+    _lib.check(st, "osem_step")
+    return act
+
+
+def rotate(image, angles, centers, background=0.0, axis_order=0):
+    """Rotation resampler (et_array_rotate), image (nz, ny, nx)."""
+    L = _lib.load()
+    img = _f32(image)
+    out = np.zeros_like(img)
+    a = np.ascontiguousarray(angles, dtype=np.float32)
+    c = np.ascontiguousarray(centers, dtype=np.float32)
+    _lib.check(L.et_array_rotate(_p(img), _size(img), _p(out), _p(a), _p(c), float(background), int(axis_order), 1),
+               "rotate")
+    return out
+
+
+def convolve(image, psf):
+    """Depth-dependent separable convolution of every z plane (et_array_convolve)."""
+    L = _lib.load()
+    img, psf = _f32(image), _f32(psf)
+    out = np.zeros_like(img)
+    _lib.check(L.et_array_convolve(_p(img), _size(img), _p(out), _size(out), _p(psf), _size(psf), 1), "convolve")
+    return out
+
+
+class Plan:
+    """Device-resident projector for torch tensors (cuda / ROCm device memory).
+
+    shape: (nz, ny, nx); cameras as for `project`; psf (nz, h, h) or None.
+    Tensors passed to the methods must be float32, contiguous, on the plan's
+    device; work is queued on torch's current stream."""
+
+    def __init__(self, shape, cameras, psf=None, background=0.0, background_attenuation=0.0):
+        L = _lib.load()
+        nz, ny, nx = shape
+        self.shape = tuple(shape)
+        cams, cols, V = _cams(cameras)
+        self.n_views = V
+        psf = _f32(psf)
+        h = psf.shape[-1] if psf is not None else 0
+        handle = ctypes.c_void_p()
+        st = L.et_plan_create(ctypes.byref(handle), nx, ny, nz, _p(cams), V, cols, _p(psf), h, float(background),
+                              float(background_attenuation))
+        _lib.check(st, "et_plan_create")
+        self._h = handle
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.et_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers ------------------------------------------------------------
+    @staticmethod
+    def _ptr(t):
+        if t is None:
+            return None
+        import torch
+        if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("expected a contiguous float32 GPU tensor")
+        return ctypes.c_void_p(t.data_ptr())
+
+    @staticmethod
+    def _stream():
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _new(self, shape, like):
+        import torch
+        return torch.empty(shape, dtype=torch.float32, device=like.device)
+
+    # -- operators ------------------------------------------------------------
+    def project(self, activity, attenuation=None, truncate=True, out=None):
+        ref = activity if activity is not None else attenuation
+        nz, ny, nx = self.shape
+        out = self._new((self.n_views, ny, nx), ref) if out is None else out
+        st = self._L.et_plan_project(self._h, self._ptr(activity), self._ptr(attenuation), self._ptr(out),
+                                     int(bool(truncate)), self._stream())
+        _lib.check(st, "et_plan_project")
+        return out
+
+    def backproject(self, sinogram, attenuation=None, mode=REFERENCE, truncate=True, out=None):
+        out = self._new(self.shape, sinogram) if out is None else out
+        st = self._L.et_plan_backproject(self._h, self._ptr(sinogram), self._ptr(attenuation), self._ptr(out),
+                                         int(mode), int(bool(truncate)), self._stream())
+        _lib.check(st, "et_plan_backproject")
+        return out
+
+    def gradient_attenuation(self, sinogram, activity, attenuation, exact=False, truncate=True, out=None):
+        out = self._new(self.shape, activity) if out is None else out
+        st = self._L.et_plan_gradient_attenuation(self._h, self._ptr(sinogram), self._ptr(activity),
+                                                  self._ptr(attenuation), self._ptr(out), int(bool(exact)),
+                                                  int(bool(truncate)), self._stream())
+        _lib.check(st, "et_plan_gradient_attenuation")
+        return out
+
+    def osem(self, activity, sinogram, attenuation=None, subsets=1, iterations=1, epsilon=1e-6):
+        """Ordered-subsets EM in place on `activity`."""
+        st = self._L.et_plan_osem(self._h, self._ptr(activity), self._ptr(sinogram), self._ptr(attenuation),
+                                  int(subsets), int(iterations), float(epsilon), self._stream())
+        _lib.check(st, "et_plan_osem")
+        return activity
+
+    def set_tuning(self, depth_chunks=0, view_batch=0):
+        _lib.check(self._L.et_plan_set_tuning(self._h, int(depth_chunks), int(view_batch)), "et_plan_set_tuning")
+
+    def profile(self, enable=True):
+        _lib.check(self._L.et_plan_profile(self._h, int(bool(enable))), "et_plan_profile")
+
+    def profile_read(self):
+        """{category: (launches, total_ms, algorithmic_bytes)}."""
+        res = {}
+        for cat, name in enumerate(_lib.PROF_NAMES):
+            n = ctypes.c_int()
+            ms = ctypes.c_double()
+            by = ctypes.c_double()
+            _lib.check(self._L.et_plan_profile_read(self._h, cat, ctypes.byref(n), ctypes.byref(ms),
+                                                    ctypes.byref(by)), "et_plan_profile_read")
+            res[name] = (n.value, ms.value, by.value)
+        return res

@@ -1,0 +1,1564 @@
+// et_host.cpp -- C++ host layer of lib_et_array_interface.so.
+//
+// Validates and expands the NiftyRec array-interface arguments, builds the
+// per-view rotation records with the reference's single-precision arithmetic,
+// owns device workspaces (plans), selects the fast (axial rotation) or general
+// path and sequences the HIP kernels of et_kernels.hip on one stream.  Every
+// HIP call is checked; errors are returned as niftyrec status codes and the
+// process is never terminated.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/et_array_interface.h"
+#include "et_internal.h"
+
+#pragma clang fp contract(off)
+
+#define ET_VERSION "niftyrec-mi355x 0.1 (gfx950)"
+
+namespace {
+
+int g_device = 0;
+std::mutex g_device_mutex;
+
+// ---------------------------------------------------------------------------
+// Status handling
+// ---------------------------------------------------------------------------
+
+struct EtError {
+    int code;
+};
+
+inline void check_hip(hipError_t e, int code)
+{
+    if (e != hipSuccess) throw EtError{code};
+}
+inline void check_launch(int e)
+{
+    if (e != (int)hipSuccess) throw EtError{niftyrec_error_kernel};
+}
+
+// Selects the library's device; returns niftyrec_error_nogpuinstalled when
+// there is no usable HIP device.
+int select_device()
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return niftyrec_error_nogpuinstalled;
+    int dev;
+    {
+        std::lock_guard<std::mutex> lk(g_device_mutex);
+        dev = g_device;
+    }
+    if (dev < 0 || dev >= count) return niftyrec_error_parameters;
+    if (hipSetDevice(dev) != hipSuccess) return niftyrec_error_nogpuinstalled;
+    return niftyrec_success;
+}
+
+// ---------------------------------------------------------------------------
+// Rotation records (single precision, reference operation order)
+// ---------------------------------------------------------------------------
+
+typedef float Mat4[16];
+
+// reg-lib/_reg_affineTransformation.cpp:18-30: R[i][j] = A[i][0]B[0][j] + ... + A[i][3]B[3][j].
+void mat_mul(const float *A, const float *B, float *R)
+{
+    float t[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            t[4 * i + j] = A[4 * i + 0] * B[0 * 4 + j] + A[4 * i + 1] * B[1 * 4 + j] +
+                           A[4 * i + 2] * B[2 * 4 + j] + A[4 * i + 3] * B[3 * 4 + j];
+    std::memcpy(R, t, sizeof(t));
+}
+
+// Centred rotation in one of six axis orders, emission-lib/_et_common.cpp:35-160.
+void rotation_matrix(float *M, float ax, float ay, float az, float cx, float cy, float cz, int order)
+{
+    const float sx = std::sin(ax), cxs = std::cos(ax), sy = std::sin(ay), cys = std::cos(ay);
+    const float sz = std::sin(az), czs = std::cos(az);
+    const float Rz[16] = {czs, -sz, 0, 0, sz, czs, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const float Ry[16] = {cys, 0, sy, 0, 0, 1, 0, 0, -sy, 0, cys, 0, 0, 0, 0, 1};
+    const float Rx[16] = {1, 0, 0, 0, 0, cxs, -sx, 0, 0, sx, cxs, 0, 0, 0, 0, 1};
+    const float Tm[16] = {0, 0, 0, cx, 0, 0, 0, cy, 0, 0, 0, cz, 0, 0, 0, 1};
+    float T1[16];
+    switch (order) {
+    case 1: mat_mul(Rz, Rx, T1); mat_mul(Ry, T1, M); break;
+    case 2: mat_mul(Rx, Ry, T1); mat_mul(Rz, T1, M); break;
+    case 3: mat_mul(Rz, Ry, T1); mat_mul(Rx, T1, M); break;
+    case 4: mat_mul(Rx, Rz, T1); mat_mul(Ry, T1, M); break;
+    case 5: mat_mul(Ry, Rz, T1); mat_mul(Rx, T1, M); break;
+    default: mat_mul(Ry, Rx, T1); mat_mul(Rz, T1, M); break;
+    }
+    float Tr[16];
+    mat_mul(M, Tm, Tr);
+    M[3] = cx - Tr[3];
+    M[7] = cy - Tr[7];
+    M[11] = cz - Tr[11];
+}
+
+// True when y' maps exactly to y: rows 1 = (0,1,0,0) and no y coupling.
+bool is_axial(const float *M)
+{
+    return M[1] == 0.f && M[9] == 0.f && M[4] == 0.f && M[5] == 1.f && M[6] == 0.f && M[7] == 0.f;
+}
+
+void inverse3(const float *M, float *out9)
+{
+    double a[9] = {M[0], M[1], M[2], M[4], M[5], M[6], M[8], M[9], M[10]};
+    double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (det == 0.0) det = 1e-30;
+    double inv[9] = {c00 / det,
+                     (a[2] * a[7] - a[1] * a[8]) / det,
+                     (a[1] * a[5] - a[2] * a[4]) / det,
+                     c01 / det,
+                     (a[0] * a[8] - a[2] * a[6]) / det,
+                     (a[2] * a[3] - a[0] * a[5]) / det,
+                     c02 / det,
+                     (a[1] * a[6] - a[0] * a[7]) / det,
+                     (a[0] * a[4] - a[1] * a[3]) / det};
+    for (int i = 0; i < 9; i++) out9[i] = (float)inv[i];
+}
+
+void make_record(const float *M, const float *Mi, float *rec)
+{
+    std::memset(rec, 0, sizeof(float) * ET_VIEW_REC);
+    std::memcpy(rec, M, sizeof(float) * 12);
+    std::memcpy(rec + 12, Mi, sizeof(float) * 12);
+    inverse3(M, rec + 24);
+}
+
+void identity_record(float *rec)
+{
+    float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    make_record(I, I, rec);
+}
+
+// Expanded cameras (3 x V, [axis*V + v]) and centres (3 x V).
+struct Cameras {
+    int V = 0;
+    std::vector<float> ang, ctr;
+};
+
+// Expansion of the cameras argument, emission-lib/_et_array_interface.cpp:396-440:
+// one column -> rotation about axis 1; six columns carry explicit centres.
+int expand_cameras(const float *cameras, int V, int cols, const int *ref_size, Cameras &out)
+{
+    if (!(cols == 1 || cols == 3 || cols == 6) || V <= 0) return niftyrec_error_parameters;
+    out.V = V;
+    out.ang.assign(3 * (size_t)V, 0.f);
+    out.ctr.assign(3 * (size_t)V, 0.f);
+    if (cols == 1)
+        for (int c = 0; c < V; c++) out.ang[V + c] = cameras[c];
+    else
+        std::memcpy(out.ang.data(), cameras, sizeof(float) * 3 * V);
+    if (cols == 6)
+        std::memcpy(out.ctr.data(), cameras + 3 * V, sizeof(float) * 3 * V);
+    else
+        for (int c = 0; c < V; c++)
+            for (int k = 0; k < 3; k++) out.ctr[k * V + c] = (float)((ref_size[k] - 1) / 2.0);
+    return niftyrec_success;
+}
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+
+struct DevBuf {
+    float *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Grows to at least `count` floats; new memory is zero-filled when `zero`.
+    float *ensure(size_t count, bool zero)
+    {
+        if (count <= n && p) return p;
+        release();
+        if (hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(float)) != hipSuccess) {
+            p = nullptr;
+            throw EtError{niftyrec_error_allocgpu};
+        }
+        n = count;
+        if (zero) check_hip(hipMemset(p, 0, count * sizeof(float)), niftyrec_error_allocgpu);
+        return p;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+
+struct et_plan {
+    int device = 0;
+    int nx = 0, ny = 0, nz = 0, V = 0;
+    int K = 0, R = 0;
+    float bg = 0.f, bga = 0.f;
+    bool fast = true;
+    std::vector<float> recs;        // V * ET_VIEW_REC
+    std::vector<float> sep_h;       // nz * 2K, convolution kernels
+    DevBuf d_recs, d_ident, d_sep, d_sep_flip;
+    // layouts
+    int gy = ET_GY, yp = 0, xp = 0, zp = 0, yu = 0, slabs = 0;
+    size_t yf_elems = 0, ustride = 0, P = 0, nvox = 0;
+    // workspaces
+    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, csum, astart, atot, tmp, tmp2, em_sino, em_vol;
+    // OSEM caches
+    DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart;
+    int osem_S = 0, osem_D = 0;
+    const float *osem_mu_key = nullptr;
+    float osem_eps = 0.f;
+    // tuning
+    int tune_D = 0, tune_vb = 0;
+    // profiling
+    bool prof = false;
+    struct Ev {
+        int cat;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+    double prof_ms[ET_PROF_COUNT] = {0}, prof_bytes[ET_PROF_COUNT] = {0};
+    int prof_n[ET_PROF_COUNT] = {0};
+    hipStream_t st = nullptr;
+
+    ~et_plan()
+    {
+        for (auto &e : events) {
+            (void)hipEventDestroy(e.a);
+            (void)hipEventDestroy(e.b);
+        }
+    }
+};
+
+namespace {
+
+int nthreads_for(int nx) { return ((nx + 63) / 64) * 64; }
+
+int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const float *psf, int psf_h,
+               float bg, float bga)
+{
+    p->nx = nx;
+    p->ny = ny;
+    p->nz = nz;
+    p->V = cams.V;
+    p->bg = bg;
+    p->bga = bga;
+    p->K = (psf && psf_h > 0) ? psf_h : 0;
+    p->R = p->K > 0 ? (p->K - 1) / 2 : 0;
+    p->recs.assign((size_t)cams.V * ET_VIEW_REC, 0.f);
+    p->fast = true;
+    for (int v = 0; v < cams.V; v++) {
+        float M[16], Mi[16];
+        const int V = cams.V;
+        rotation_matrix(M, cams.ang[v], cams.ang[V + v], cams.ang[2 * V + v], cams.ctr[v], cams.ctr[V + v],
+                        cams.ctr[2 * V + v], 0);
+        rotation_matrix(Mi, -cams.ang[v], -cams.ang[V + v], -cams.ang[2 * V + v], cams.ctr[v], cams.ctr[V + v],
+                        cams.ctr[2 * V + v], 5);
+        make_record(M, Mi, &p->recs[(size_t)v * ET_VIEW_REC]);
+        if (!is_axial(M) || !is_axial(Mi)) p->fast = false;
+    }
+    // the fast kernels sample with zero padding; a non-zero background takes the
+    // general path, whose resampler reproduces the reference padding rule
+    if (p->R > ET_RMAX || bg != 0.f || bga != 0.f) p->fast = false;
+    // layouts
+    p->slabs = (ny + ET_T - 1) / ET_T;
+    p->xp = nx + 2;
+    p->zp = nz + 2;
+    p->yp = ET_GY + p->slabs * ET_T + ET_GY;
+    p->yu = p->slabs * ET_T;
+    p->yf_elems = (size_t)p->yp * p->xp * p->zp;
+    p->ustride = (size_t)p->yu * p->xp * p->zp;
+    p->P = (size_t)nx * ny;
+    p->nvox = (size_t)nx * ny * nz;
+    // device copies
+    p->d_recs.ensure(p->recs.size(), false);
+    check_hip(hipMemcpy(p->d_recs.p, p->recs.data(), sizeof(float) * p->recs.size(), hipMemcpyHostToDevice),
+              niftyrec_error_transfergpu);
+    std::vector<float> ident(ET_VIEW_REC);
+    identity_record(ident.data());
+    p->d_ident.ensure(ET_VIEW_REC, false);
+    check_hip(hipMemcpy(p->d_ident.p, ident.data(), sizeof(float) * ET_VIEW_REC, hipMemcpyHostToDevice),
+              niftyrec_error_transfergpu);
+    if (p->K > 0) {
+        // PSF separation, emission-lib/_et.cpp:192-215: row kernel = central row,
+        // column kernel = central column / centre value.
+        const int h = p->K, c = (h - 1) / 2;
+        p->sep_h.assign((size_t)nz * 2 * h, 0.f);
+        std::vector<float> flip((size_t)nz * 2 * h);
+        for (int n = 0; n < nz; n++) {
+            const float *pl = psf + (size_t)h * h * n;
+            const float norm = pl[c * h + c];
+            for (int i = 0; i < h; i++) {
+                p->sep_h[(size_t)2 * h * n + i] = pl[c * h + i];
+                p->sep_h[(size_t)2 * h * n + h + i] = pl[c + i * h] / norm;
+            }
+            for (int i = 0; i < h; i++) {
+                flip[(size_t)2 * h * n + i] = p->sep_h[(size_t)2 * h * n + (h - 1 - i)];
+                flip[(size_t)2 * h * n + h + i] = p->sep_h[(size_t)2 * h * n + h + (h - 1 - i)];
+            }
+        }
+        p->d_sep.ensure(p->sep_h.size(), false);
+        p->d_sep_flip.ensure(flip.size(), false);
+        check_hip(hipMemcpy(p->d_sep.p, p->sep_h.data(), sizeof(float) * p->sep_h.size(), hipMemcpyHostToDevice),
+                  niftyrec_error_transfergpu);
+        check_hip(hipMemcpy(p->d_sep_flip.p, flip.data(), sizeof(float) * flip.size(), hipMemcpyHostToDevice),
+                  niftyrec_error_transfergpu);
+    }
+    return niftyrec_success;
+}
+
+// ---- profiling --------------------------------------------------------------
+
+struct ProfScope {
+    et_plan *p;
+    int cat;
+    double bytes;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(et_plan *p_, int cat_, double bytes_) : p(p_), cat(cat_), bytes(bytes_)
+    {
+        if (!p->prof) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            p->prof = false;
+            return;
+        }
+        (void)hipEventRecord(a, p->st);
+    }
+    ~ProfScope()
+    {
+        if (!p->prof || !a) return;
+        (void)hipEventRecord(b, p->st);
+        p->events.push_back({cat, bytes, a, b});
+    }
+};
+
+void prof_collect(et_plan *p)
+{
+    if (p->events.empty()) return;
+    (void)hipStreamSynchronize(p->st);
+    for (auto &e : p->events) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.a, e.b) == hipSuccess) {
+            p->prof_ms[e.cat] += ms;
+            p->prof_bytes[e.cat] += e.bytes;
+            p->prof_n[e.cat] += 1;
+        }
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    p->events.clear();
+}
+
+// ---- sweep helpers ------------------------------------------------------------
+
+EtSweepArgs base_args(et_plan *p)
+{
+    EtSweepArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.nz = p->nz;
+    a.yp = p->yp;
+    a.xp = p->xp;
+    a.gy = p->gy;
+    a.yu = p->yu;
+    a.ustride = (int)p->ustride;
+    a.slabs = p->slabs;
+    a.K = p->K;
+    a.psf = p->K > 0 ? p->d_sep.p : nullptr;
+    a.bg_act = p->bg;
+    a.bg_mu = p->bga;
+    a.bg_line = p->bg;
+    a.eps = 0.f;
+    a.D = 1;
+    a.zlen = p->nz;
+    return a;
+}
+
+// Depth chunks so that a launch over `views` views fills the device
+// (about 4 waves per SIMD of 256 CUs), each chunk at least 8 planes deep.
+int choose_D(et_plan *p, int views)
+{
+    if (p->tune_D > 0) return std::min(p->tune_D, p->nz);
+    const int waves_per_block = nthreads_for(p->nx) / 64;
+    const int ts = et_sweep_slab_rows(ET_MODE_PROJECT, p->R);
+    const long slabs = (p->ny + ts - 1) / ts;
+    const long target = 4L * 1024;
+    const long waves = slabs * views * waves_per_block;
+    int D = 1;
+    while (waves * D < target && p->nz / (D * 2) >= 8) D *= 2;
+    return D;
+}
+
+int view_batch(et_plan *p)
+{
+    if (p->tune_vb > 0) return std::min(p->tune_vb, p->V);
+    const size_t bytes = p->ustride * sizeof(float);
+    long vb = (long)((256ull << 20) / std::max<size_t>(bytes, 1));
+    return (int)std::max(1L, std::min<long>(vb, p->V));
+}
+
+void pack(et_plan *p, const float *abi, DevBuf &yf)
+{
+    yf.ensure(p->yf_elems, true);
+    ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
+    check_launch(et_launch_pack_yf(abi, yf.p, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
+}
+
+// Attenuation at the start of every depth chunk (D > 1) and along whole rays
+// for `nv` views whose records start at `recs`; results in p->astart / p->atot.
+void mu_prefix(et_plan *p, const float *recs, int nv, int D, const float *mu_yf, bool want_start, bool want_tot,
+               float *astart_out = nullptr, float *atot_out = nullptr)
+{
+    EtSweepArgs a = base_args(p);
+    a.views = recs;
+    a.V = nv;
+    a.D = D;
+    a.zlen = (p->nz + D - 1) / D;
+    a.mu = mu_yf;
+    p->csum.ensure((size_t)nv * D * p->P, false);
+    a.part = p->csum.p;
+    {
+        ProfScope ps(p, ET_PROF_MUSUM, (double)nv * (4.0 * p->nvox + 4.0 * D * p->P));
+        check_launch(et_launch_sweep(ET_MODE_MUSUM, 0, 1, 0, a, (et_stream_t)p->st));
+    }
+    float *as = nullptr, *at = nullptr;
+    if (want_start) as = astart_out ? astart_out : p->astart.ensure((size_t)nv * D * p->P, false);
+    if (want_tot) at = atot_out ? atot_out : p->atot.ensure((size_t)nv * p->P, false);
+    ProfScope ps(p, ET_PROF_OTHER, 0);
+    check_launch(et_launch_chunk_scan(p->csum.p, as, at, nv, D, (int)p->P, (et_stream_t)p->st));
+}
+
+// Fast-path projection of nv views (records recs) into sino [nv][ny][nx].
+void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *sino,
+                  int truncate, int D, const float *astart, float *parts_out = nullptr)
+{
+    EtSweepArgs a = base_args(p);
+    a.views = recs;
+    a.V = nv;
+    a.D = D;
+    a.zlen = (p->nz + D - 1) / D;
+    a.act = act_yf;
+    a.mu = mu_yf;
+    a.astart = astart;
+    float *part = parts_out;
+    if (!part) part = (D == 1) ? sino : p->part.ensure((size_t)D * nv * p->P, false);
+    a.part = part;
+    a.truncate = (D == 1 && !parts_out) ? truncate : 0;
+    {
+        const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
+        ProfScope ps(p, ET_PROF_PROJECT, bytes);
+        check_launch(et_launch_sweep(ET_MODE_PROJECT, p->R, mu_yf != nullptr, 1, a, (et_stream_t)p->st));
+    }
+    if (D > 1 && !parts_out) {
+        ProfScope ps(p, ET_PROF_OTHER, 0);
+        check_launch(et_launch_reduce_parts(part, D, nv, (int)p->P, sino, truncate, (et_stream_t)p->st));
+    }
+}
+
+// u for nv views: PSF(s e^{-A}) (flip: correlation kernels).
+void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
+              const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
+              float *u_out)
+{
+    EtSweepArgs a = base_args(p);
+    a.views = recs;
+    a.V = nv;
+    a.D = D;
+    a.zlen = (p->nz + D - 1) / D;
+    a.mu = mu_yf;
+    a.astart = astart;
+    a.src = src;
+    a.sino = sino;
+    a.meas = meas;
+    a.rpart = rpart;
+    a.rD = rD;
+    a.eps = eps;
+    a.u = u_out;
+    if (p->K > 0) a.psf = flip ? p->d_sep_flip.p : p->d_sep.p;
+    const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
+    ProfScope ps(p, ET_PROF_BPU, bytes);
+    check_launch(et_launch_sweep(ET_MODE_BPU, p->R, mu_yf != nullptr, 0, a, (et_stream_t)p->st));
+}
+
+void pull_fast(et_plan *p, const float *recs, int nv, const float *u, bool adjoint, int epi, bool accumulate,
+               bool truncate, float *out, float *x = nullptr, const float *sens = nullptr)
+{
+    EtPullArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.nz = p->nz;
+    a.yu = p->yu;
+    a.xp = p->xp;
+    a.ustride = (int)p->ustride;
+    a.V = nv;
+    a.views = recs;
+    a.u = u;
+    a.bg = p->bg;
+    a.adjoint = adjoint;
+    a.epilogue = epi;
+    a.accumulate = accumulate;
+    a.truncate = truncate;
+    a.out = out;
+    a.yp = p->yp;
+    a.gy = p->gy;
+    a.x = x;
+    a.sens = sens;
+    const double bytes = (double)nv * 4.0 * p->nvox + (accumulate ? 8.0 : 4.0) * p->nvox;
+    ProfScope ps(p, ET_PROF_PULL, bytes);
+    check_launch(et_launch_pull(a, (et_stream_t)p->st));
+}
+
+// ---- general path: one view at a time ------------------------------------------
+
+// Rotated copy of an ABI volume for view v in the YF layout (+ PSF when the
+// radius is too large for the sweep kernels).
+void rotate_general(et_plan *p, const float *abi, int v, float bg, DevBuf &dst, bool psf, bool flip)
+{
+    dst.ensure(p->yf_elems, true);
+    {
+        ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
+        check_launch(et_launch_resample_general(abi, p->nx, p->ny, p->nz, p->d_recs.p + (size_t)v * ET_VIEW_REC, bg,
+                                                dst.p, 1, p->yp, p->gy, (et_stream_t)p->st));
+    }
+    if (psf && p->K > 0 && p->R > ET_RMAX) {
+        // row (x) pass then column (y) pass, emission-lib/_et_convolveSeparable2D.cpp:43-75
+        p->tmp.ensure(p->yf_elems, true);
+        const long sx = p->yp, sy = 1, sz = (long)p->yp * p->xp, off = p->gy + (long)p->yp * (1 + p->xp);
+        const float *sep = flip ? p->d_sep_flip.p : p->d_sep.p;
+        check_launch(et_launch_conv_axis(dst.p, p->tmp.p, p->nx, p->ny, p->nz, sx, sy, sz, off, 0, sep, p->K, 0,
+                                         (et_stream_t)p->st));
+        check_launch(et_launch_conv_axis(p->tmp.p, dst.p, p->nx, p->ny, p->nz, sx, sy, sz, off, 1, sep, p->K, 0,
+                                         (et_stream_t)p->st));
+    }
+}
+
+// Sweep radius used on the general path (0 when the PSF is applied separately).
+int general_R(et_plan *p) { return p->R > ET_RMAX ? 0 : p->R; }
+
+void conv_u_general(et_plan *p, float *u, bool flip)
+{
+    if (!(p->K > 0 && p->R > ET_RMAX)) return;
+    p->tmp2.ensure(p->ustride, true);
+    const long sx = p->yu, sy = 1, sz = (long)p->yu * p->xp, off = (long)p->yu * (1 + p->xp);
+    const float *sep = flip ? p->d_sep_flip.p : p->d_sep.p;
+    if (!flip) {
+        check_launch(et_launch_conv_axis(u, p->tmp2.p, p->nx, p->ny, p->nz, sx, sy, sz, off, 0, sep, p->K, 0,
+                                         (et_stream_t)p->st));
+        check_launch(et_launch_conv_axis(p->tmp2.p, u, p->nx, p->ny, p->nz, sx, sy, sz, off, 1, sep, p->K, 0,
+                                         (et_stream_t)p->st));
+    } else {
+        check_launch(et_launch_conv_axis(u, p->tmp2.p, p->nx, p->ny, p->nz, sx, sy, sz, off, 1, sep, p->K, 0,
+                                         (et_stream_t)p->st));
+        check_launch(et_launch_conv_axis(p->tmp2.p, u, p->nx, p->ny, p->nz, sx, sy, sz, off, 0, sep, p->K, 0,
+                                         (et_stream_t)p->st));
+    }
+}
+
+void pull_general(et_plan *p, int v, const float *u, bool adjoint, bool accumulate, bool truncate, float *out,
+                  int epi)
+{
+    EtPullArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = p->nx;
+    a.ny = p->ny;
+    a.nz = p->nz;
+    a.yu = p->yu;
+    a.xp = p->xp;
+    a.ustride = (int)p->ustride;
+    a.V = 1;
+    a.views = p->d_recs.p + (size_t)v * ET_VIEW_REC;
+    a.u = u;
+    a.bg = p->bg;
+    a.adjoint = adjoint;
+    a.epilogue = epi;
+    a.accumulate = accumulate;
+    a.truncate = truncate;
+    a.out = out;
+    a.yp = p->yp;
+    a.gy = p->gy;
+    ProfScope ps(p, ET_PROF_PULL, 4.0 * p->nvox + (accumulate ? 8.0 : 4.0) * p->nvox);
+    check_launch(et_launch_pull_general(a, (et_stream_t)p->st));
+}
+
+// ---------------------------------------------------------------------------
+// Operations (device pointers, ABI layouts)
+// ---------------------------------------------------------------------------
+
+void op_project(et_plan *p, const float *act, const float *mu, float *sino, int truncate)
+{
+    const float *act_yf = nullptr, *mu_yf = nullptr;
+    if (p->fast) {
+        if (act) {
+            pack(p, act, p->act_yf);
+            act_yf = p->act_yf.p;
+        }
+        if (mu) {
+            pack(p, mu, p->mu_yf);
+            mu_yf = p->mu_yf.p;
+        }
+        if (!act) {
+            mu_prefix(p, p->d_recs.p, p->V, 1, mu_yf, false, true);
+            ProfScope ps(p, ET_PROF_OTHER, 0);
+            check_launch(et_launch_bg_only(p->atot.p, p->bg, (int)(p->V * p->P), sino, truncate, (et_stream_t)p->st));
+            return;
+        }
+        const int D = mu ? choose_D(p, p->V) : std::max(1, choose_D(p, p->V));
+        const float *astart = nullptr;
+        if (mu && D > 1) {
+            mu_prefix(p, p->d_recs.p, p->V, D, mu_yf, true, false);
+            astart = p->astart.p;
+        }
+        project_fast(p, p->d_recs.p, p->V, act_yf, mu_yf, sino, truncate, D, astart);
+        return;
+    }
+    // general path
+    const int R = general_R(p);
+    for (int v = 0; v < p->V; v++) {
+        float *sv = sino + (size_t)v * p->P;
+        if (mu) rotate_general(p, mu, v, p->bga, p->rot_mu, false, false);
+        if (!act) {
+            mu_prefix(p, p->d_ident.p, 1, 1, p->rot_mu.p, false, true);
+            check_launch(et_launch_bg_only(p->atot.p, p->bg, (int)p->P, sv, truncate, (et_stream_t)p->st));
+            continue;
+        }
+        rotate_general(p, act, v, p->bg, p->rot_act, true, false);
+        EtSweepArgs a = base_args(p);
+        a.views = p->d_ident.p;
+        a.V = 1;
+        a.act = p->rot_act.p;
+        a.mu = mu ? p->rot_mu.p : nullptr;
+        a.bg_act = 0.f;
+        a.bg_mu = 0.f;
+        a.part = sv;
+        a.truncate = truncate;
+        ProfScope ps(p, ET_PROF_PROJECT, (mu ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
+        check_launch(et_launch_sweep(ET_MODE_PROJECT, R, mu != nullptr, 1, a, (et_stream_t)p->st));
+    }
+}
+
+void op_backproject(et_plan *p, const float *sino, const float *mu, float *out, bool adjoint, int truncate)
+{
+    if (p->fast) {
+        const float *mu_yf = nullptr;
+        if (mu) {
+            pack(p, mu, p->mu_yf);
+            mu_yf = p->mu_yf.p;
+        }
+        const int vb = view_batch(p);
+        p->u.ensure((size_t)vb * p->ustride, true);
+        for (int v0 = 0; v0 < p->V; v0 += vb) {
+            const int nv = std::min(vb, p->V - v0);
+            const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
+            const int D = choose_D(p, nv);
+            const float *astart = nullptr;
+            if (mu && D > 1) {
+                mu_prefix(p, recs, nv, D, mu_yf, true, false);
+                astart = p->astart.p;
+            }
+            bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, sino + (size_t)v0 * p->P, nullptr, nullptr, 0, 0.f, D, astart,
+                     adjoint, p->u.p);
+            const bool last = v0 + nv >= p->V;
+            pull_fast(p, recs, nv, p->u.p, adjoint, ET_EPI_ABI_STORE, v0 > 0, last && truncate, out);
+        }
+        return;
+    }
+    const int R = general_R(p);
+    p->u.ensure(p->ustride, true);
+    for (int v = 0; v < p->V; v++) {
+        if (mu) rotate_general(p, mu, v, p->bga, p->rot_mu, false, false);
+        EtSweepArgs a = base_args(p);
+        a.views = p->d_ident.p;
+        a.V = 1;
+        a.mu = mu ? p->rot_mu.p : nullptr;
+        a.bg_mu = 0.f;
+        a.src = ET_SRC_SINO;
+        a.sino = sino + (size_t)v * p->P;
+        a.u = p->u.p;
+        if (p->K > 0) a.psf = adjoint ? p->d_sep_flip.p : p->d_sep.p;
+        {
+            ProfScope ps(p, ET_PROF_BPU, (mu ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
+            check_launch(et_launch_sweep(ET_MODE_BPU, R, mu != nullptr, 0, a, (et_stream_t)p->st));
+        }
+        conv_u_general(p, p->u.p, adjoint);
+        pull_general(p, v, p->u.p, adjoint, v > 0, truncate && v == p->V - 1, out, ET_EPI_ABI_STORE);
+    }
+}
+
+void op_gradient(et_plan *p, const float *sino, const float *act, const float *mu, float *out, bool exact,
+                 int truncate)
+{
+    const int mode = exact ? ET_MODE_GRAD_MU : ET_MODE_GRAD_REF;
+    if (p->fast) {
+        pack(p, act, p->act_yf);
+        pack(p, mu, p->mu_yf);
+        const int vb = view_batch(p);
+        p->u.ensure((size_t)vb * p->ustride, true);
+        for (int v0 = 0; v0 < p->V; v0 += vb) {
+            const int nv = std::min(vb, p->V - v0);
+            const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
+            mu_prefix(p, recs, nv, 1, p->mu_yf.p, false, true);
+            EtSweepArgs a = base_args(p);
+            a.views = recs;
+            a.V = nv;
+            a.act = p->act_yf.p;
+            a.mu = p->mu_yf.p;
+            a.atot = p->atot.p;
+            a.src = ET_SRC_SINO;
+            a.sino = sino + (size_t)v0 * p->P;
+            a.u = p->u.p;
+            {
+                ProfScope ps(p, ET_PROF_GRAD, (double)nv * (12.0 * p->nvox + 4.0 * p->P));
+                check_launch(et_launch_sweep(mode, p->R, 1, 1, a, (et_stream_t)p->st));
+            }
+            const bool last = v0 + nv >= p->V;
+            pull_fast(p, recs, nv, p->u.p, exact, ET_EPI_ABI_STORE, v0 > 0, last && truncate, out);
+        }
+        return;
+    }
+    const int R = general_R(p);
+    p->u.ensure(p->ustride, true);
+    for (int v = 0; v < p->V; v++) {
+        rotate_general(p, mu, v, p->bga, p->rot_mu, false, false);
+        rotate_general(p, act, v, p->bg, p->rot_act, exact, false);
+        mu_prefix(p, p->d_ident.p, 1, 1, p->rot_mu.p, false, true);
+        EtSweepArgs a = base_args(p);
+        a.views = p->d_ident.p;
+        a.V = 1;
+        a.act = p->rot_act.p;
+        a.mu = p->rot_mu.p;
+        a.bg_act = 0.f;
+        a.bg_mu = 0.f;
+        a.atot = p->atot.p;
+        a.src = ET_SRC_SINO;
+        a.sino = sino + (size_t)v * p->P;
+        a.u = p->u.p;
+        {
+            ProfScope ps(p, ET_PROF_GRAD, 12.0 * p->nvox + 4.0 * p->P);
+            check_launch(et_launch_sweep(mode, R, 1, 1, a, (et_stream_t)p->st));
+        }
+        if (!exact) conv_u_general(p, p->u.p, false);
+        pull_general(p, v, p->u.p, exact, v > 0, truncate && v == p->V - 1, out, ET_EPI_ABI_STORE);
+    }
+}
+
+// MLEM update restricted to views [v0, v0+nv) of the plan with every volume in
+// the YF layout; the reference update of _et_array_interface.cpp:1080-1131.
+// sens_yf must hold the clamped sensitivity of these views.
+void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const float *mu_yf, const float *meas,
+                      const float *sens_yf, float eps, int D, const float *astart)
+{
+    p->u.ensure((size_t)nv * p->ustride, true);
+    float *parts = p->part.ensure((size_t)D * nv * p->P, false);
+    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, D, astart, parts);
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_RATIO, nullptr, meas, parts, D, eps, D, astart, false, p->u.p);
+    pull_fast(p, recs, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, x_yf, sens_yf);
+}
+
+// Sensitivity of views [recs, nv): reference backprojection of ones, truncated,
+// then clamped below at eps (_et_array_interface.cpp:1112-1121), into sens_yf.
+void em_sensitivity(et_plan *p, const float *recs, int nv, const float *mu_yf, float eps, int D,
+                    const float *astart, float *sens_yf)
+{
+    p->u.ensure((size_t)nv * p->ustride, true);
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, astart, false, p->u.p);
+    pull_fast(p, recs, nv, p->u.p, false, ET_EPI_YF_STORE, false, true, sens_yf);
+    ProfScope ps(p, ET_PROF_OTHER, 0);
+    check_launch(et_launch_yf_clamp_min(sens_yf, p->nx, p->ny, p->nz, p->yp, p->gy, eps, (et_stream_t)p->st));
+}
+
+// General-path MLEM step over all views of the plan (ABI layouts), the
+// reference update of _et_array_interface.cpp:1080-1131.
+void em_step_general(et_plan *p, float *x, const float *meas, const float *mu, float eps)
+{
+    const size_t Ps = p->P * p->V;
+    float *proj = p->em_sino.ensure(Ps, false);
+    op_project(p, x, mu, proj, 1);
+    {
+        ProfScope ps(p, ET_PROF_OTHER, 0);
+        check_launch(et_launch_ratio(meas, proj, proj, Ps, eps, (et_stream_t)p->st));
+    }
+    float *b = p->em_vol.ensure(2 * p->nvox, false);
+    float *s = b + p->nvox;
+    op_backproject(p, proj, mu, b, false, 1);
+    check_launch(et_launch_fill(proj, Ps, 1.f, (et_stream_t)p->st));
+    op_backproject(p, proj, mu, s, false, 1);
+    check_launch(et_launch_clamp_min(s, p->nvox, eps, (et_stream_t)p->st));
+    check_launch(et_launch_em_update_flat(x, b, s, p->nvox, (et_stream_t)p->st));
+}
+
+void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, int iters, float eps)
+{
+    if (S <= 0 || S > p->V) throw EtError{niftyrec_error_parameters};
+    if (!p->fast) {
+        if (S != 1) throw EtError{niftyrec_error_parameters};
+        for (int it = 0; it < iters; it++) em_step_general(p, x, meas, mu, eps);
+        return;
+    }
+    const float *mu_yf = nullptr;
+    if (mu) {
+        pack(p, mu, p->mu_yf);
+        mu_yf = p->mu_yf.p;
+    }
+    // subset s: views s, s+S, ...; records and measured planes reordered by subset
+    std::vector<int> order, start(S + 1, 0);
+    for (int s = 0; s < S; s++) {
+        start[s] = (int)order.size();
+        for (int v = s; v < p->V; v += S) order.push_back(v);
+    }
+    start[S] = (int)order.size();
+    const int maxsub = (p->V + S - 1) / S;
+    const int D = choose_D(p, maxsub);
+    const bool rebuild = p->osem_S != S || p->osem_D != D || p->osem_mu_key != mu || p->osem_eps != eps;
+    if (rebuild) {
+        std::vector<float> recs((size_t)p->V * ET_VIEW_REC);
+        for (int i = 0; i < p->V; i++)
+            std::memcpy(&recs[(size_t)i * ET_VIEW_REC], &p->recs[(size_t)order[i] * ET_VIEW_REC],
+                        sizeof(float) * ET_VIEW_REC);
+        p->osem_recs.ensure(recs.size(), false);
+        check_hip(hipMemcpyAsync(p->osem_recs.p, recs.data(), sizeof(float) * recs.size(), hipMemcpyHostToDevice,
+                                 p->st),
+                  niftyrec_error_transfergpu);
+        p->sens.ensure((size_t)S * p->yf_elems, true);
+        if (mu_yf && D > 1) p->osem_astart.ensure((size_t)p->V * D * p->P, false);
+        for (int s = 0; s < S; s++) {
+            const int nv = start[s + 1] - start[s];
+            const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
+            float *as = nullptr;
+            if (mu_yf && D > 1) {
+                as = p->osem_astart.p + (size_t)start[s] * D * p->P;
+                mu_prefix(p, r, nv, D, mu_yf, true, false, as);
+            }
+            em_sensitivity(p, r, nv, mu_yf, eps, D, as, p->sens.p + (size_t)s * p->yf_elems);
+        }
+        check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
+        p->osem_S = S;
+        p->osem_D = D;
+        p->osem_mu_key = mu;
+        p->osem_eps = eps;
+    }
+    // measured data in subset order (device to device copies)
+    p->osem_meas.ensure((size_t)p->V * p->P, false);
+    for (int i = 0; i < p->V; i++)
+        check_hip(hipMemcpyAsync(p->osem_meas.p + (size_t)i * p->P, meas + (size_t)order[i] * p->P,
+                                 sizeof(float) * p->P, hipMemcpyDeviceToDevice, p->st),
+                  niftyrec_error_transfergpu);
+    pack(p, x, p->x_yf);
+    for (int it = 0; it < iters; it++)
+        for (int s = 0; s < S; s++) {
+            const int nv = start[s + 1] - start[s];
+            const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
+            const float *as = (mu_yf && D > 1) ? p->osem_astart.p + (size_t)start[s] * D * p->P : nullptr;
+            em_subset_update(p, r, nv, p->x_yf.p, mu_yf, p->osem_meas.p + (size_t)start[s] * p->P,
+                             p->sens.p + (size_t)s * p->yf_elems, eps, D, as);
+        }
+    ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
+    check_launch(et_launch_unpack_yf(p->x_yf.p, x, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
+}
+
+// ---------------------------------------------------------------------------
+// Host-pointer calls: upload, run on a private stream, download.
+// ---------------------------------------------------------------------------
+
+struct HostCall {
+    hipStream_t st = nullptr;
+    std::vector<float *> bufs;
+    ~HostCall()
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        for (float *b : bufs) (void)hipFree(b);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    void init()
+    {
+        check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), niftyrec_error_unspecified);
+    }
+    float *alloc(size_t n)
+    {
+        float *p = nullptr;
+        if (hipMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(float)) != hipSuccess)
+            throw EtError{niftyrec_error_allocgpu};
+        bufs.push_back(p);
+        return p;
+    }
+    float *upload(const float *h, size_t n)
+    {
+        float *d = alloc(n);
+        check_hip(hipMemcpyAsync(d, h, n * sizeof(float), hipMemcpyHostToDevice, st), niftyrec_error_transfergpu);
+        return d;
+    }
+    void download(float *h, const float *d, size_t n)
+    {
+        check_hip(hipMemcpyAsync(h, d, n * sizeof(float), hipMemcpyDeviceToHost, st), niftyrec_error_transfergpu);
+        check_hip(hipStreamSynchronize(st), niftyrec_error_kernel);
+    }
+};
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        return f();
+    } catch (const EtError &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        return niftyrec_error_alloccpu;
+    } catch (...) {
+        return niftyrec_error_unspecified;
+    }
+}
+
+bool is_absent(const int *s) { return s == nullptr || (s[0] == 0 && s[1] == 0 && s[2] == 0); }
+
+int project_host(float *activity, int *activity_size, float *sinogram, int *sinogram_size, float *cameras,
+                 int *cameras_size, float *psf, int *psf_size, float *attenuation, int *attenuation_size,
+                 float background, float background_attenuation, int truncate)
+{
+    // Validation, emission-lib/_et_array_interface.cpp:357-393.
+    const bool no_act = is_absent(activity_size) || activity == nullptr;
+    const bool no_psf = is_absent(psf_size) || psf == nullptr;
+    const bool no_mu = is_absent(attenuation_size) || attenuation == nullptr;
+    if (no_act && no_mu) return niftyrec_error_parameters;
+    if (no_act && activity_size) {  // reference quirk: activity_size takes the attenuation size (:360)
+        activity_size[0] = attenuation_size[0];
+        activity_size[1] = attenuation_size[1];
+        activity_size[2] = attenuation_size[2];
+    }
+    const int *vs = no_act ? attenuation_size : activity_size;
+    const int nx = vs[0], ny = vs[1], nz = vs[2];
+    const int V = cameras_size[0], cols = cameras_size[1];
+    if (!(cols == 1 || cols == 3 || cols == 6)) return niftyrec_error_parameters;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || V <= 0) return niftyrec_error_parameters;
+    if (!no_psf) {
+        if (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != nx) return niftyrec_error_parameters;
+        if (psf_size[0] != psf_size[1] || nz != nx) return niftyrec_error_parameters;
+    }
+    if (!no_act && !no_mu && (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
+        return niftyrec_error_parameters;
+    if (sinogram_size && (sinogram_size[0] != nx || sinogram_size[1] != ny || sinogram_size[2] != V))
+        return niftyrec_error_parameters;
+    if (nx > 1024) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        Cameras cams;
+        int rc = expand_cameras(cameras, V, cols, vs, cams);
+        if (rc) return rc;
+        et_plan plan;
+        HostCall hc;
+        hc.init();
+        plan.st = hc.st;
+        rc = build_plan(&plan, nx, ny, nz, cams, no_psf ? nullptr : psf, no_psf ? 0 : psf_size[0], background,
+                        background_attenuation);
+        if (rc) return rc;
+        const size_t n = (size_t)nx * ny * nz;
+        const float *d_act = no_act ? nullptr : hc.upload(activity, n);
+        const float *d_mu = no_mu ? nullptr : hc.upload(attenuation, n);
+        float *d_sino = hc.alloc((size_t)nx * ny * V);
+        op_project(&plan, d_act, d_mu, d_sino, truncate);
+        hc.download(sinogram, d_sino, (size_t)nx * ny * V);
+        return niftyrec_success;
+    });
+}
+
+int backproject_host(float *sino, int *sino_size, float *bkpr, int *bkpr_size, float *cameras, int *cameras_size,
+                     float *psf, int *psf_size, float *attenuation, int *attenuation_size, float background,
+                     float background_attenuation, int truncate, bool adjoint)
+{
+    // Validation, emission-lib/_et_array_interface.cpp:565-600.
+    const bool no_psf = is_absent(psf_size) || psf == nullptr;
+    const bool no_mu = is_absent(attenuation_size) || attenuation == nullptr;
+    const int V = cameras_size[0], cols = cameras_size[1];
+    if (!(cols == 1 || cols == 3 || cols == 6)) return niftyrec_error_parameters;
+    if (sino_size[2] != V || V <= 0) return niftyrec_error_parameters;
+    const int nx = bkpr_size[0], ny = bkpr_size[1], nz = bkpr_size[2];
+    if (nx != sino_size[0] || ny != sino_size[1] || nz <= 0 || nx <= 0 || ny <= 0) return niftyrec_error_parameters;
+    if (!no_psf) {
+        if (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != sino_size[0])
+            return niftyrec_error_parameters;
+        if (psf_size[0] != psf_size[1] || nz != nx) return niftyrec_error_parameters;
+    }
+    if (!no_mu && (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
+        return niftyrec_error_parameters;
+    if (nx > 1024) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        // centres from activity size [N, m, N] (_et_array_interface.cpp:598-600)
+        const int act_size[3] = {sino_size[0], sino_size[1], sino_size[0]};
+        Cameras cams;
+        int rc = expand_cameras(cameras, V, cols, act_size, cams);
+        if (rc) return rc;
+        et_plan plan;
+        HostCall hc;
+        hc.init();
+        plan.st = hc.st;
+        rc = build_plan(&plan, nx, ny, nz, cams, no_psf ? nullptr : psf, no_psf ? 0 : psf_size[0], background,
+                        background_attenuation);
+        if (rc) return rc;
+        const size_t n = (size_t)nx * ny * nz;
+        const float *d_sino = hc.upload(sino, (size_t)nx * ny * V);
+        const float *d_mu = no_mu ? nullptr : hc.upload(attenuation, n);
+        float *d_out = hc.alloc(n);
+        op_backproject(&plan, d_sino, d_mu, d_out, adjoint, truncate);
+        hc.download(bkpr, d_out, n);
+        return niftyrec_success;
+    });
+}
+
+// One MLEM update on host arrays with cameras [3 x V] (the EM drivers).
+int em_step_host(float *activity, int size_x, int size_y, const float *sino, int V, const float *cams3,
+                 const float *attenuation, const float *psf, int psf_h, float bg, float bga, float eps)
+{
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        const int nx = size_x, ny = size_y, nz = size_x;
+        const int vs[3] = {nx, ny, nz};
+        Cameras cams;
+        int rc = expand_cameras(cams3, V, 3, vs, cams);
+        if (rc) return rc;
+        et_plan plan;
+        HostCall hc;
+        hc.init();
+        plan.st = hc.st;
+        rc = build_plan(&plan, nx, ny, nz, cams, psf, psf ? psf_h : 0, bg, bga);
+        if (rc) return rc;
+        const size_t n = (size_t)nx * ny * nz;
+        float *d_x = hc.upload(activity, n);
+        const float *d_meas = hc.upload(sino, (size_t)nx * ny * V);
+        const float *d_mu = attenuation ? hc.upload(attenuation, n) : nullptr;
+        op_osem(&plan, d_x, d_meas, d_mu, 1, 1, eps);
+        hc.download(activity, d_x, n);
+        return niftyrec_success;
+    });
+}
+
+thread_local char t_msg[2048];
+
+}  // namespace
+
+// ===========================================================================
+// Exported C ABI
+// ===========================================================================
+
+extern "C" {
+
+int status_success(int *status)
+{
+    *status = STATUS_SUCCESS;
+    return STATUS_SUCCESS;
+}
+int status_io_error(int *status)
+{
+    *status = STATUS_IO_ERROR;
+    return STATUS_SUCCESS;
+}
+int status_initialisation_error(int *status)
+{
+    *status = STATUS_INITIALISATION_ERROR;
+    return STATUS_SUCCESS;
+}
+int status_parameter_error(int *status)
+{
+    *status = STATUS_PARAMETER_ERROR;
+    return STATUS_SUCCESS;
+}
+int status_unhandled_error(int *status)
+{
+    *status = STATUS_UNHANDLED_ERROR;
+    return STATUS_SUCCESS;
+}
+int echo(int *in, int *out)
+{
+    *out = *in;
+    return STATUS_SUCCESS;
+}
+
+char *niftyrec_error_msg(int status)
+{
+    const char *m;
+    switch (status) {
+    case niftyrec_success: m = "niftyrec: OK.\n"; break;
+    case niftyrec_error_unspecified: m = "niftyrec: ERROR - unspecified error.\n"; break;
+    case niftyrec_error_parameters: m = "niftyrec: ERROR - inconsistent parameters.\n"; break;
+    case niftyrec_error_kernel: m = "niftyrec: ERROR - gpu kernel execution error.\n"; break;
+    case niftyrec_error_allocgpu: m = "niftyrec: ERROR - GPU memory allocation.\n"; break;
+    case niftyrec_error_alloccpu: m = "niftyrec: ERROR - cpu memory allocation.\n"; break;
+    case niftyrec_error_transfergpu: m = "niftyrec: ERROR - transferring memory between host and GPU.\n"; break;
+    case niftyrec_error_nogpubuilt: m = "niftyrec: ERROR - built without GPU support.\n"; break;
+    case niftyrec_error_nogpuinstalled: m = "niftyrec: ERROR - no GPUs installed in the system.\n"; break;
+    default: m = "niftyrec: unknown status.\n"; break;
+    }
+    std::snprintf(t_msg, sizeof(t_msg), "%s", m);
+    return t_msg;
+}
+
+const char *et_library_version(void) { return ET_VERSION; }
+
+int et_array_list_gpus(int *gpu_count, int *gpus_info_array)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        *gpu_count = 0;
+        return niftyrec_error_nogpuinstalled;
+    }
+    const int cmax = std::min(count, 16);
+    for (int d = 0; d < cmax; d++) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) return niftyrec_error_unspecified;
+        // {id, gflops proxy (CUs * clock), CUs, clock kHz, MB}, reference emission-lib/_et.cpp:2004-2018
+        gpus_info_array[5 * d + 0] = d;
+        gpus_info_array[5 * d + 1] = prop.multiProcessorCount * prop.clockRate;
+        gpus_info_array[5 * d + 2] = prop.multiProcessorCount;
+        gpus_info_array[5 * d + 3] = prop.clockRate;
+        gpus_info_array[5 * d + 4] = (int)std::floor(prop.totalGlobalMem / 1000000.0);
+    }
+    *gpu_count = count;
+    return niftyrec_success;
+}
+
+int et_array_set_gpu(int id)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return niftyrec_error_nogpuinstalled;
+    if (id < 0 || id >= count) return niftyrec_error_parameters;
+    std::lock_guard<std::mutex> lk(g_device_mutex);
+    g_device = id;
+    return niftyrec_success;
+}
+
+int et_array_set_gpu_pointer(int *id) { return et_array_set_gpu(*id); }
+
+int et_array_isinstalled(void) { return 0; }
+
+int et_array_reset_gpu(void)
+{
+    int st = select_device();
+    if (st) return st;
+    return hipDeviceSynchronize() == hipSuccess ? niftyrec_success : niftyrec_error_kernel;
+}
+
+unsigned short int et_array_is_block_multiple(unsigned short int size)
+{
+    (void)size;
+    return 1;
+}
+
+unsigned short int et_array_get_block_size(void) { return 1; }
+
+int SPECT_project_parallelholes(float *activity, int *activity_size, float *sinogram, int *sinogram_size,
+                                float *cameras, int *cameras_size, float *psf, int *psf_size, float *attenuation,
+                                int *attenuation_size, float *Background, float *Background_attenuation, int *Gpu,
+                                int *Truncate_negative_values)
+{
+    (void)Gpu;
+    return project_host(activity, activity_size, sinogram, sinogram_size, cameras, cameras_size, psf, psf_size,
+                        attenuation, attenuation_size, *Background, *Background_attenuation,
+                        *Truncate_negative_values);
+}
+
+int SPECT_backproject_parallelholes(float *sino, int *sino_size, float *bkpr, int *bkpr_size, float *cameras,
+                                    int *cameras_size, float *psf, int *psf_size, float *attenuation,
+                                    int *attenuation_size, float *Background, float *Background_attenuation, int *Gpu,
+                                    int *Truncate_negative_values)
+{
+    (void)Gpu;
+    return backproject_host(sino, sino_size, bkpr, bkpr_size, cameras, cameras_size, psf, psf_size, attenuation,
+                            attenuation_size, *Background, *Background_attenuation, *Truncate_negative_values, false);
+}
+
+int et_array_project(float *activity, int *activity_size, float *sinogram, int *sinogram_size, float *cameras,
+                     int *cameras_size, float *psf, int *psf_size, float *attenuation, int *attenuation_size,
+                     float background, float background_attenuation, int GPU, int truncate_negative_values)
+{
+    (void)GPU;
+    return project_host(activity, activity_size, sinogram, sinogram_size, cameras, cameras_size, psf, psf_size,
+                        attenuation, attenuation_size, background, background_attenuation, truncate_negative_values);
+}
+
+int et_array_backproject(float *sino, int *sino_size, float *bkpr, int *bkpr_size, float *cameras, int *cameras_size,
+                         float *psf, int *psf_size, float *attenuation, int *attenuation_size, float background,
+                         float background_attenuation, int GPU, int truncate_negative_values)
+{
+    (void)GPU;
+    return backproject_host(sino, sino_size, bkpr, bkpr_size, cameras, cameras_size, psf, psf_size, attenuation,
+                            attenuation_size, background, background_attenuation, truncate_negative_values, false);
+}
+
+int et_array_gradient_attenuation(float *sino, int *sino_size, float *activity, int *activity_size, float *gradient,
+                                  int *gradient_size, float *cameras, int *cameras_size, float *psf, int *psf_size,
+                                  float *attenuation, int *attenuation_size, float background,
+                                  float background_attenuation, int GPU, int truncate_negative_values)
+{
+    (void)GPU;
+    // Validation, emission-lib/_et_array_interface.cpp:727-780 (the 2-D branch
+    // applies when the sinogram holds one plane).
+    const bool no_psf = is_absent(psf_size) || psf == nullptr;
+    const int V = cameras_size[0], cols = cameras_size[1];
+    if (!(cols == 1 || cols == 3 || cols == 6) || V <= 0) return niftyrec_error_parameters;
+    if (sino_size[2] == 1) {
+        if (sino_size[1] != V) return niftyrec_error_parameters;
+        if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] != sino_size[0])) return niftyrec_error_parameters;
+    } else {
+        if (sino_size[2] != V) return niftyrec_error_parameters;
+        if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != sino_size[0]))
+            return niftyrec_error_parameters;
+    }
+    const int nx = gradient_size[0], ny = gradient_size[1], nz = gradient_size[2];
+    if (nx != sino_size[0] || ny != sino_size[1] || nz <= 0) return niftyrec_error_parameters;
+    if (activity == nullptr || attenuation == nullptr) return niftyrec_error_parameters;
+    if (!no_psf && (psf_size[0] != psf_size[1] || nz != nx)) return niftyrec_error_parameters;
+    if (activity_size && (activity_size[0] != nx || activity_size[1] != ny || activity_size[2] != nz))
+        return niftyrec_error_parameters;
+    if (attenuation_size && !is_absent(attenuation_size) &&
+        (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
+        return niftyrec_error_parameters;
+    if (nx > 1024) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        Cameras cams;
+        int rc = expand_cameras(cameras, V, cols, activity_size ? activity_size : gradient_size, cams);
+        if (rc) return rc;
+        et_plan plan;
+        HostCall hc;
+        hc.init();
+        plan.st = hc.st;
+        rc = build_plan(&plan, nx, ny, nz, cams, no_psf ? nullptr : psf, no_psf ? 0 : psf_size[0], background,
+                        background_attenuation);
+        if (rc) return rc;
+        const size_t n = (size_t)nx * ny * nz;
+        const float *d_sino = hc.upload(sino, (size_t)nx * ny * V);
+        const float *d_act = hc.upload(activity, n);
+        const float *d_mu = hc.upload(attenuation, n);
+        float *d_out = hc.alloc(n);
+        op_gradient(&plan, d_sino, d_act, d_mu, d_out, false, truncate_negative_values);
+        hc.download(gradient, d_out, n);
+        return niftyrec_success;
+    });
+}
+
+int et_array_rotate(float *image, int *size, float *rotated_image, float *angles, float *centers, float background,
+                    int axis_order, int GPU)
+{
+    (void)GPU;
+    if (size[0] <= 0 || size[1] <= 0 || size[2] <= 0) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        float M[16];
+        rotation_matrix(M, angles[0], angles[1], angles[2], centers[0], centers[1], centers[2], axis_order);
+        HostCall hc;
+        hc.init();
+        const size_t n = (size_t)size[0] * size[1] * size[2];
+        const float *d_in = hc.upload(image, n);
+        float *d_M = hc.upload(M, 12);
+        float *d_out = hc.alloc(n);
+        check_launch(et_launch_resample_general(d_in, size[0], size[1], size[2], d_M, background, d_out, 0, 0, 0,
+                                                (et_stream_t)hc.st));
+        hc.download(rotated_image, d_out, n);
+        return niftyrec_success;
+    });
+}
+
+int et_array_affine(float *image, int *image_size, float *transformed_image, int *transformed_image_size,
+                    float *affine, int *affine_size, float background, int GPU)
+{
+    (void)GPU;
+    (void)affine_size;
+    if (image_size[0] <= 0 || image_size[1] <= 0 || image_size[2] <= 0) return niftyrec_error_parameters;
+    if (transformed_image_size && (transformed_image_size[0] != image_size[0] ||
+                                   transformed_image_size[1] != image_size[1] ||
+                                   transformed_image_size[2] != image_size[2]))
+        return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        HostCall hc;
+        hc.init();
+        const size_t n = (size_t)image_size[0] * image_size[1] * image_size[2];
+        const float *d_in = hc.upload(image, n);
+        float *d_M = hc.upload(affine, 12);
+        float *d_out = hc.alloc(n);
+        check_launch(et_launch_resample_general(d_in, image_size[0], image_size[1], image_size[2], d_M, background,
+                                                d_out, 0, 0, 0, (et_stream_t)hc.st));
+        hc.download(transformed_image, d_out, n);
+        return niftyrec_success;
+    });
+}
+
+int et_array_convolve(float *image, int *image_size, float *out, int *out_size, float *psf, int *psf_size, int GPU)
+{
+    (void)GPU;
+    const int nx = image_size[0], ny = image_size[1], nz = image_size[2];
+    if (nx <= 0 || ny <= 0 || nz <= 0) return niftyrec_error_parameters;
+    if (out_size && (out_size[0] != nx || out_size[1] != ny || out_size[2] != nz)) return niftyrec_error_parameters;
+    if (psf_size[0] % 2 != 1 || psf_size[0] != psf_size[1] || psf_size[2] != nz) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        HostCall hc;
+        hc.init();
+        const int h = psf_size[0], c = (h - 1) / 2;
+        std::vector<float> sep((size_t)nz * 2 * h);
+        for (int n = 0; n < nz; n++) {
+            const float *pl = psf + (size_t)h * h * n;
+            const float norm = pl[c * h + c];
+            for (int i = 0; i < h; i++) {
+                sep[(size_t)2 * h * n + i] = pl[c * h + i];
+                sep[(size_t)2 * h * n + h + i] = pl[c + i * h] / norm;
+            }
+        }
+        const size_t n = (size_t)nx * ny * nz;
+        const float *d_in = hc.upload(image, n);
+        const float *d_sep = hc.upload(sep.data(), sep.size());
+        float *d_tmp = hc.alloc(n);
+        float *d_out = hc.alloc(n);
+        check_launch(et_launch_conv_axis(d_in, d_tmp, nx, ny, nz, 1, nx, (long)nx * ny, 0, 0, d_sep, h, 0,
+                                         (et_stream_t)hc.st));
+        check_launch(et_launch_conv_axis(d_tmp, d_out, nx, ny, nz, 1, nx, (long)nx * ny, 0, 1, d_sep, h, 0,
+                                         (et_stream_t)hc.st));
+        hc.download(out, d_out, n);
+        return niftyrec_success;
+    });
+}
+
+int et_array_calculate_size_psf(unsigned int *psf_size_x, unsigned int *psf_size_y, float fwhm_pixels_dist0,
+                                float sensitivity0, float dist0, float fwhm_pixels_dist1, float sensitivity1,
+                                float dist1)
+{
+    (void)fwhm_pixels_dist0; (void)sensitivity0; (void)dist0;
+    (void)fwhm_pixels_dist1; (void)sensitivity1; (void)dist1;
+    *psf_size_x = 5;
+    *psf_size_y = 5;
+    return 0;
+}
+
+int et_array_make_psf(float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y, float fwhm_pixels_dist0,
+                      float sensitivity0, float dist0, float fwhm_pixels_dist1, float sensitivity1, float dist1,
+                      unsigned int N_psf_planes)
+{
+    unsigned int sx, sy;
+    et_array_calculate_size_psf(&sx, &sy, fwhm_pixels_dist0, sensitivity0, dist0, fwhm_pixels_dist1, sensitivity1,
+                                dist1);
+    if (psf_size_x != sx || psf_size_y != sy) return 1;
+    for (size_t i = 0; i < (size_t)psf_size_x * psf_size_y * N_psf_planes; i++) psf_data[i] = 1.f;
+    return 0;
+}
+
+// Inclusive angular endpoints, emission-lib/_et_array_interface.cpp:923-937.
+int et_array_make_cameras(float *cameras_data, float firstcamera_deg, float lastcamera_deg, unsigned int n_cameras,
+                          unsigned int rotation_axis)
+{
+    const float PI_F = 3.141592653589793f;
+    for (unsigned int cam = 0; cam < n_cameras; cam++) {
+        const float angle =
+            (firstcamera_deg + (lastcamera_deg - firstcamera_deg) * cam / (n_cameras - 1)) / 180.0f * PI_F;
+        for (unsigned int axis = 0; axis < 3; axis++)
+            cameras_data[cam + axis * n_cameras] = (axis == rotation_axis) ? angle : 0.0f;
+    }
+    return 0;
+}
+
+int et_array_mlem_step(float *activity_data, int size_x, int size_y, float *sinogram_data, int n_cameras,
+                       float *cameras_data, int use_attenuation, float *attenuation_data, int use_psf,
+                       float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y, float background,
+                       float background_attenuation, float epsilon, int use_gpu)
+{
+    (void)use_gpu;
+    if (size_x <= 0 || size_y <= 0 || n_cameras <= 0) return niftyrec_error_parameters;
+    if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
+    if (size_x > 1024) return niftyrec_error_parameters;
+    return em_step_host(activity_data, size_x, size_y, sinogram_data, n_cameras, cameras_data,
+                        use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr,
+                        (int)psf_size_x, background, background_attenuation, epsilon);
+}
+
+// Random camera subset drawn with replacement from glibc rand(), as
+// emission-lib/_et_array_interface.cpp:981-1009 does (two rand() calls per
+// camera, R = base + fine/scale in single precision); an index equal to
+// n_cameras (the reference's clamp) is clamped to n_cameras - 1 here.
+int et_array_osem_step(float *activity_data, int size_x, int size_y, unsigned int subset_order, float *sinogram_data,
+                       int n_cameras, float *cameras_data, int use_attenuation, float *attenuation_data, int use_psf,
+                       float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y, float background,
+                       float background_attenuation, float epsilon, int use_gpu)
+{
+    (void)use_gpu;
+    if (size_x <= 0 || size_y <= 0 || n_cameras <= 0 || subset_order == 0) return niftyrec_error_parameters;
+    const int nsub = n_cameras / (int)subset_order;
+    if (nsub <= 0) return niftyrec_error_parameters;
+    if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
+    const size_t P = (size_t)size_x * size_y;
+    std::vector<float> cams((size_t)3 * nsub), sino_sub(P * nsub);
+    const float scale = (float)(RAND_MAX + 1.);
+    for (int c = 0; c < nsub; c++) {
+        const float base = rand() / scale;
+        const float fine = rand() / scale;
+        const float Rr = base + fine / scale;
+        unsigned int idx = (unsigned int)(n_cameras * Rr);
+        if (idx >= (unsigned int)n_cameras) idx = n_cameras - 1;
+        for (int axis = 0; axis < 3; axis++) cams[c + axis * nsub] = cameras_data[idx + axis * n_cameras];
+        std::memcpy(&sino_sub[(size_t)c * P], sinogram_data + (size_t)idx * P, sizeof(float) * P);
+    }
+    return em_step_host(activity_data, size_x, size_y, sino_sub.data(), nsub, cams.data(),
+                        use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr,
+                        (int)psf_size_x, background, background_attenuation, epsilon);
+}
+
+int et_array_mlem(float *activity_data, unsigned int size_x, unsigned int size_y, float *sinogram_data, int n_cameras,
+                  float firstcamera, float lastcamera, unsigned int rotation_axis, int iterations, int use_attenuation,
+                  float *attenuation_data, int use_psf, float *psf_data, unsigned int psf_size_x,
+                  unsigned int psf_size_y, float background, float background_attenuation, float epsilon, int use_gpu)
+{
+    for (size_t i = 0; i < (size_t)size_x * size_y * size_x; i++) activity_data[i] = 1.0f;
+    std::vector<float> cams((size_t)3 * n_cameras);
+    int status = et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
+    for (int it = 0; it < iterations; it++)
+        status += et_array_mlem_step(activity_data, size_x, size_y, sinogram_data, n_cameras, cams.data(),
+                                     use_attenuation, attenuation_data, use_psf, psf_data, psf_size_x, psf_size_y,
+                                     background, background_attenuation, epsilon, use_gpu);
+    return status;
+}
+
+int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y, unsigned int subset_order,
+                  float *sinogram_data, int n_cameras, float firstcamera, float lastcamera,
+                  unsigned int rotation_axis, int iterations, int use_attenuation, float *attenuation_data,
+                  int use_psf, float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y, float background,
+                  float background_attenuation, float epsilon, int use_gpu)
+{
+    for (size_t i = 0; i < (size_t)size_x * size_y * size_x; i++) activity_data[i] = 1.0f;
+    std::vector<float> cams((size_t)3 * n_cameras);
+    int status = et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
+    for (int it = 0; it < iterations; it++)
+        status += et_array_osem_step(activity_data, size_x, size_y, subset_order, sinogram_data, n_cameras,
+                                     cams.data(), use_attenuation, attenuation_data, use_psf, psf_data, psf_size_x,
+                                     psf_size_y, background, background_attenuation, epsilon, use_gpu);
+    return status;
+}
+
+// ---- device-resident plans ------------------------------------------------------
+
+int et_plan_create(et_plan **plan, int nx, int ny, int nz, const float *cameras, int n_cameras, int camera_cols,
+                   const float *psf, int psf_h, float background, float background_attenuation)
+{
+    if (!plan) return niftyrec_error_parameters;
+    *plan = nullptr;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 1024 || n_cameras <= 0) return niftyrec_error_parameters;
+    if (psf && (psf_h % 2 != 1 || nz != nx)) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        const int vs[3] = {nx, ny, nz};
+        Cameras cams;
+        int rc = expand_cameras(cameras, n_cameras, camera_cols, vs, cams);
+        if (rc) return rc;
+        et_plan *p = new et_plan();
+        (void)hipGetDevice(&p->device);
+        try {
+            rc = build_plan(p, nx, ny, nz, cams, psf, psf ? psf_h : 0, background, background_attenuation);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        if (rc) {
+            delete p;
+            return rc;
+        }
+        *plan = p;
+        return niftyrec_success;
+    });
+}
+
+int et_plan_destroy(et_plan *plan)
+{
+    if (!plan) return niftyrec_error_parameters;
+    (void)hipSetDevice(plan->device);
+    if (plan->st) (void)hipStreamSynchronize(plan->st);
+    delete plan;
+    return niftyrec_success;
+}
+
+#define ET_PLAN_CALL(plan, stream, body)                                        \
+    if (!plan) return niftyrec_error_parameters;                                \
+    if (hipSetDevice(plan->device) != hipSuccess) return niftyrec_error_nogpuinstalled; \
+    plan->st = reinterpret_cast<hipStream_t>(stream);                           \
+    return guarded([&]() -> int { body; return niftyrec_success; });
+
+int et_plan_project(et_plan *plan, const float *act, const float *mu, float *sino, int truncate, void *stream)
+{
+    if (!act && !mu) return niftyrec_error_parameters;
+    ET_PLAN_CALL(plan, stream, op_project(plan, act, mu, sino, truncate))
+}
+
+int et_plan_backproject(et_plan *plan, const float *sino, const float *mu, float *out, int adjoint, int truncate,
+                        void *stream)
+{
+    ET_PLAN_CALL(plan, stream, op_backproject(plan, sino, mu, out, adjoint != 0, truncate))
+}
+
+int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *act, const float *mu, float *out,
+                                 int exact, int truncate, void *stream)
+{
+    if (!act || !mu) return niftyrec_error_parameters;
+    ET_PLAN_CALL(plan, stream, op_gradient(plan, sino, act, mu, out, exact != 0, truncate))
+}
+
+int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, int subsets, int iterations,
+                 float epsilon, void *stream)
+{
+    ET_PLAN_CALL(plan, stream, op_osem(plan, act, sino, mu, subsets, iterations, epsilon))
+}
+
+int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch_)
+{
+    if (!plan) return niftyrec_error_parameters;
+    plan->tune_D = depth_chunks;
+    plan->tune_vb = view_batch_;
+    plan->osem_S = 0;  // cached OSEM data depend on the chunking
+    return niftyrec_success;
+}
+
+int et_plan_profile(et_plan *plan, int enable)
+{
+    if (!plan) return niftyrec_error_parameters;
+    prof_collect(plan);
+    plan->prof = enable != 0;
+    for (int i = 0; i < ET_PROF_COUNT; i++) {
+        plan->prof_ms[i] = 0;
+        plan->prof_bytes[i] = 0;
+        plan->prof_n[i] = 0;
+    }
+    return niftyrec_success;
+}
+
+int et_plan_profile_read(et_plan *plan, int category, int *launches, double *total_ms, double *bytes)
+{
+    if (!plan || category < 0 || category >= ET_PROF_COUNT) return niftyrec_error_parameters;
+    prof_collect(plan);
+    if (launches) *launches = plan->prof_n[category];
+    if (total_ms) *total_ms = plan->prof_ms[category];
+    if (bytes) *bytes = plan->prof_bytes[category];
+    return niftyrec_success;
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// et_internal.h -- declarations shared by the HIP kernels (et_kernels.hip) and the
+// C++ host layer (et_host.cpp) of lib_et_array_interface.so.
+//
+// Device data layouts
+// -------------------
+// ABI layout   [nx, ny, nz] x fastest: idx = x + nx*(y + ny*z)   (caller buffers)
+// YF layout    axial-fastest, zero-padded source volumes read by the sweep kernels:
+//              idx = (gy + y) + yp*((x + 1) + xp*(z + 1)),
+//              xp = nx + 2, zp = nz + 2, yp = gy + roundup(ny, ET_T) + gy.
+//              Everything outside the logical volume is zero, so trilinear
+//              corners that fall one voxel outside the grid read zeros.
+// U layout     rotated-frame per-view volumes written by the sweep kernels and
+//              pulled by the backprojection kernels:
+//              idx = v*ustride + y + yu*((x' + 1) + xp*(z' + 1)), yu = roundup(ny, ET_T),
+//              with a zero border of one voxel in x' and z'.
+// Sinogram     [nx, ny, V] x fastest (the ABI layout), partial projections
+//              [D][V][ny][nx].
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#define ET_T 16      // axial rows per slab in the sweep and pull kernels
+#define ET_GY 16     // guard rows of the YF layout (>= roundup(R, 4))
+#define ET_RMAX 10   // largest PSF radius handled inside the sweep kernels (K <= 21)
+#define ET_VIEW_REC 40  // floats per view record (see EtViewRec)
+
+// Per-view record in device memory (ET_VIEW_REC floats):
+//   [0..11]  forward 3x4 matrix M (row-major): sample position = M (x,y,z,1)
+//   [12..23] reference backprojection matrix (negated angles, ZYX order)
+//   [24..32] inverse of the 3x3 linear part of M (row-major), used to bound the
+//            candidate search of the exact-adjoint gather
+//   [33..39] unused
+enum EtSweepMode {
+    ET_MODE_PROJECT = 0,   // p = sum_z PSF(act') e^{-A} (+ bg e^{-A_tot})
+    ET_MODE_BPU = 1,       // u = PSF(s e^{-A})  (kernels flipped by the host for the adjoint)
+    ET_MODE_GRAD_REF = 2,  // u = PSF(s sum_{k>=z} act' e^{-A})        (reverse sweep)
+    ET_MODE_GRAD_MU = 3,   // u = -s (sum_{k>=z} PSF(act') e^{-A} + bg e^{-A_tot})  (reverse sweep)
+    ET_MODE_MUSUM = 4,     // per-chunk sums of mu' along each ray
+};
+
+enum EtSource {
+    ET_SRC_SINO = 0,   // s read from a [V][ny][nx] array
+    ET_SRC_RATIO = 1,  // s = ratio(meas, sum_d part[d]) with the MLEM epsilon rule
+    ET_SRC_ONES = 2,   // s = 1 (sensitivity)
+};
+
+struct EtSweepArgs {
+    int nx, ny, nz;
+    int yp, xp, gy;          // YF layout of act/mu
+    int yu, ustride;         // U layout of the output volumes (per view stride)
+    int V, D, zlen;          // views in launch, depth chunks, z' per chunk
+    int slabs;               // ceil(ny / ET_T)
+    const float *views;      // [V][ET_VIEW_REC]
+    const float *act;        // YF
+    const float *mu;         // YF or null
+    const float *psf;        // [nz][2K] row kernel, column kernel (null when R == 0)
+    int K;
+    float bg_act, bg_mu;     // padding used when resampling act / mu
+    float bg_line;           // background term of the line integral
+    const float *astart;     // [V][D][ny][nx] attenuation at the start of each chunk (D > 1)
+    const float *atot;       // [V][ny][nx] total attenuation along each ray (reverse sweeps)
+    float *part;             // PROJECT: [D][V][ny][nx]; MUSUM: [V][D][ny][nx]
+    int truncate;            // PROJECT with D == 1: clamp negatives
+    int src;                 // EtSource
+    const float *sino;       // ET_SRC_SINO: [V][ny][nx]
+    const float *meas;       // ET_SRC_RATIO: measured counts [V][ny][nx]
+    const float *rpart;      // ET_SRC_RATIO: partial projections [D_r][V][ny][nx]
+    int rD;
+    float eps;
+    float *u;                // U layout outputs
+};
+
+enum EtEpilogue {
+    ET_EPI_ABI_STORE = 0,    // out_abi  = acc (optionally + previous, truncated at the end)
+    ET_EPI_YF_STORE = 1,     // out_yf   = acc (optionally + previous)
+    ET_EPI_OSEM = 2,         // x_yf    *= max(acc,0) / sens_yf   (sens already clamped)
+};
+
+struct EtPullArgs {
+    int nx, ny, nz;
+    int yu, xp, ustride;     // U layout of the inputs
+    int V;
+    const float *views;      // [V][ET_VIEW_REC]
+    const float *u;
+    float bg;                // padding of the reference inverse-rotation pull
+    int adjoint;             // 0: reference pull, 1: exact transpose of the forward pull
+    int epilogue;            // EtEpilogue
+    int accumulate;          // add the previous content of the output
+    int truncate;            // clamp negatives after this launch
+    float *out;              // ABI or YF output
+    int yp, gy;              // YF layout of out / x / sens
+    float *x;                // ET_EPI_OSEM: activity (YF), updated in place
+    const float *sens;       // ET_EPI_OSEM: clamped sensitivity (YF)
+};
+
+// ---- launchers (et_kernels.hip) -------------------------------------------------
+// All return hipSuccess or the launch error; all are asynchronous on `stream`.
+typedef struct ihipStream_t *et_stream_t;
+
+int et_launch_sweep(int mode, int R, int has_mu, int has_act, const EtSweepArgs &a,
+                    et_stream_t stream);
+int et_launch_pull(const EtPullArgs &a, et_stream_t stream);
+int et_launch_pull_general(const EtPullArgs &a, et_stream_t stream);
+int et_launch_pack_yf(const float *abi, float *yf, int nx, int ny, int nz, int yp, int gy,
+                      et_stream_t stream);
+int et_launch_unpack_yf(const float *yf, float *abi, int nx, int ny, int nz, int yp, int gy,
+                        et_stream_t stream);
+int et_launch_resample_general(const float *src, int nx, int ny, int nz, const float *M12,
+                               float bg, float *dst, int dst_yf, int yp, int gy,
+                               et_stream_t stream);
+int et_launch_chunk_scan(const float *csum, float *astart, float *atot, int V, int D, int P,
+                         et_stream_t stream);
+int et_launch_reduce_parts(const float *part, int D, int V, int P, float *sino, int truncate,
+                           et_stream_t stream);
+int et_launch_bg_only(const float *atot, float bg, int n, float *sino, int truncate,
+                      et_stream_t stream);
+int et_launch_conv_axis(const float *src, float *dst, int nx, int ny, int nz, long sx, long sy,
+                        long sz, long off, int axis, const float *sep, int K, int flip,
+                        et_stream_t stream);
+int et_launch_truncate(float *buf, size_t n, et_stream_t stream);
+int et_launch_fill(float *buf, size_t n, float value, et_stream_t stream);
+int et_launch_clamp_min(float *buf, size_t n, float lo, et_stream_t stream);
+int et_launch_yf_clamp_min(float *yf, int nx, int ny, int nz, int yp, int gy, float lo,
+                           et_stream_t stream);
+int et_launch_osem_yf_update(float *x_yf, const float *b_yf, const float *s_yf, int nx, int ny,
+                             int nz, int yp, int gy, et_stream_t stream);
+int et_launch_em_update_flat(float *x, const float *b, const float *s, size_t n, et_stream_t stream);
+int et_launch_ratio(const float *meas, const float *proj, float *out, size_t n, float eps,
+                    et_stream_t stream);
+// Axial rows per block of a sweep launch (mode, PSF radius).
+int et_sweep_slab_rows(int mode, int R);
+// Sweep kernel variants compiled in (for diagnostics).
+int et_sweep_variant_count(void);

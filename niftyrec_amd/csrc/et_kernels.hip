@@ -1,0 +1,1205 @@
+// et_kernels.hip -- HIP kernels of the SPECT projector for gfx950 (CDNA4).
+//
+// Fast path (every view a rotation about the axial y axis, which is what
+// one-column cameras produce, reference emission-lib/_et_array_interface.cpp:414-419):
+// the sample position of rotated voxel (x', y, z') is (px(x',z'), y, pz(x',z')),
+// so the bilinear weights and corner addresses are computed once per (x', z')
+// and reused for a whole run of axial rows, which are contiguous in the YF
+// layout.  One workgroup owns a (view, axial slab of ET_T rows, depth chunk) and
+// sweeps depth z': resample -> depth-dependent separable PSF (column pass in
+// registers, row pass through LDS) -> attenuation e^{-A} with A the running
+// inclusive sum of the resampled mu -> accumulate.  Nothing rotated is ever
+// stored for the projection.
+//
+// General path (any other rotation, a PSF wider than ET_RMAX, or a non-zero
+// background, whose padding rule k_resample_general reproduces): the rotated
+// volumes are materialised per view and swept with an identity view record.
+//
+// No MFMA: nothing here is a dense contraction.  Kernels never allocate.
+#include <hip/hip_runtime.h>
+
+#include "et_internal.h"
+
+namespace {
+
+// Keeps the scheduler from hoisting the loads of every 4-row chunk of a run
+// ahead of the arithmetic (which multiplies the register footprint and
+// halves occupancy); latency is hidden by the other waves of the SIMD instead.
+#ifndef ET_NO_CHUNK_FENCE
+#define ET_CHUNK_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define ET_CHUNK_FENCE()
+#endif
+
+constexpr int T = ET_T;
+
+// ------------------------------------------------------------------------------
+// Arithmetic helpers
+// ------------------------------------------------------------------------------
+
+// Sample position with the reference's rounding: ((m0*x + m1*y) + m2*z) + m3,
+// reg-lib/_reg_affineTransformation.cpp:42-49.  For fast views m1 == 0 so the
+// y term is an exact +-0 and is dropped.
+__device__ __forceinline__ float pos2(float m0, float x, float m2, float z, float m3)
+{
+#pragma clang fp contract(off)
+    return (m0 * x + m2 * z) + m3;
+}
+
+__device__ __forceinline__ float pos3(const float *m, float x, float y, float z)
+{
+#pragma clang fp contract(off)
+    return ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
+}
+
+// e^{-A} for A >= 0.  x = -A log2(e) is formed as a rounded product plus its
+// exact residual (fma) and the low part of log2(e); 2^x = 2^n 2^f with
+// n = rint(x), |f| <= 1/2, 2^f from the hardware exp2 (v_exp_f32).
+__device__ __forceinline__ float exp_neg(float A)
+{
+    const float L2E = 1.44269502162933349609375f;   // float(log2 e)
+    const float L2E_LO = 1.925963033500011e-08f;    // log2(e) - L2E
+    float x = -A * L2E;
+    float r = fmaf(-A, L2E, -x);
+    r = fmaf(-A, L2E_LO, r);
+    float n = rintf(x);
+    float f = (x - n) + r;
+    return ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+}
+
+// Bilinear sample in the (x, z) plane shared by a run of axial rows.
+struct Samp {
+    int off;                        // element offset of corner (x0, z0) in a padded layout
+    float w00, w01, w10, w11;       // weights of (x0,z0), (x0+1,z0), (x0,z0+1), (x0+1,z0+1)
+    float xb0, xb1;                 // x basis (kept for the background rule)
+    bool vx0, vx1, vz0, vz1;        // corner validity against the logical grid
+};
+
+// px, pz: sample position; strides dx, dz of the padded layout whose padded
+// index of grid coordinate c is c + 1 (so corners one voxel outside read 0).
+// Weight rule: reference reg-lib/_reg_resampling.cpp:345-364.
+__device__ __forceinline__ Samp make_samp(float px, float pz, int nx, int nz, int dx, int dz)
+{
+    Samp s;
+    float fx = floorf(px), fz = floorf(pz);
+    float rx = px - fx, rz = pz - fz;
+    s.xb0 = 1.f - rx;
+    s.xb1 = rx;
+    float zb0 = 1.f - rz, zb1 = rz;
+    s.vx0 = (fx >= 0.f) && (fx <= (float)(nx - 1));
+    s.vx1 = (fx >= -1.f) && (fx <= (float)(nx - 2));
+    s.vz0 = (fz >= 0.f) && (fz <= (float)(nz - 1));
+    s.vz1 = (fz >= -1.f) && (fz <= (float)(nz - 2));
+    bool inside = (fx >= -1.f) && (fx <= (float)(nx - 1)) && (fz >= -1.f) && (fz <= (float)(nz - 1));
+    int x0 = inside ? (int)fx : -1;
+    int z0 = inside ? (int)fz : -1;
+    float m = inside ? 1.f : 0.f;
+    s.off = dx * (x0 + 1) + dz * (z0 + 1);
+    s.w00 = m * (s.xb0 * zb0);
+    s.w01 = m * (s.xb1 * zb0);
+    s.w10 = m * (s.xb0 * zb1);
+    s.w11 = m * (s.xb1 * zb1);
+    return s;
+}
+
+// Streams the bilinear samples of NA consecutive rows (starting SA rows into a
+// LA-long aligned run at p00) through the axial (column) pass of the PSF:
+//   yc[i] = sum_{j=0}^{2R} row[i+j] * kc[2R-j],  i < TS
+// (convolution out[y] = sum_k in[y+k] kc[R-k], _et_convolveSeparable2D.cpp:28-39).
+// Each loaded chunk of 4 rows is consumed at once, so raw rows never stay live.
+// F(j, value) -> value maps a resampled row before it enters the pass.
+template <int TS, int R, int NA, int LA, int SA, class F>
+__device__ __forceinline__ void resample_column(const float *__restrict__ p00, int dx, int dz, const Samp &s,
+                                                const float *__restrict__ kc, F f, float *yc)
+{
+    const float4 *q00 = reinterpret_cast<const float4 *>(p00);
+    const float4 *q01 = reinterpret_cast<const float4 *>(p00 + dx);
+    const float4 *q10 = reinterpret_cast<const float4 *>(p00 + dz);
+    const float4 *q11 = reinterpret_cast<const float4 *>(p00 + dz + dx);
+#pragma unroll
+    for (int i = 0; i < TS; i++) yc[i] = 0.f;
+#pragma unroll
+    for (int k = 0; k < LA / 4; k++) {
+        if (4 * k + 3 < SA || 4 * k >= SA + NA) continue;
+        const float4 a = q00[k], b = q01[k], c = q10[k], d = q11[k];
+        const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+        const float cv[4] = {c.x, c.y, c.z, c.w}, dv[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int j = 4 * k + e - SA;
+            if (j < 0 || j >= NA) continue;
+            const float v = f(j, av[e] * s.w00 + bv[e] * s.w01 + cv[e] * s.w10 + dv[e] * s.w11);
+            if constexpr (R == 0) {
+                yc[j] = v;
+            } else {
+#pragma unroll
+                for (int i = 0; i < TS; i++) {
+                    const int t = j - i;  // tap index into the reversed kernel
+                    if (t >= 0 && t <= 2 * R) yc[i] = fmaf(v, kc[2 * R - t], yc[i]);
+                }
+            }
+        }
+        ET_CHUNK_FENCE();
+    }
+}
+
+// out[j] = bilinear sample of rows SKIP + j of a LOADED-long aligned run, j < NROWS,
+// mapped through F(j, value).
+template <int NROWS, int LOADED, int SKIP, class F>
+__device__ __forceinline__ void bilinear_map(const float *__restrict__ p00, int dx, int dz, const Samp &s, F f)
+{
+    const float4 *q00 = reinterpret_cast<const float4 *>(p00);
+    const float4 *q01 = reinterpret_cast<const float4 *>(p00 + dx);
+    const float4 *q10 = reinterpret_cast<const float4 *>(p00 + dz);
+    const float4 *q11 = reinterpret_cast<const float4 *>(p00 + dz + dx);
+#pragma unroll
+    for (int k = 0; k < LOADED / 4; k++) {
+        if (4 * k + 3 < SKIP || 4 * k >= SKIP + NROWS) continue;
+        const float4 a = q00[k], b = q01[k], c = q10[k], d = q11[k];
+        const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+        const float cv[4] = {c.x, c.y, c.z, c.w}, dv[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int j = 4 * k + e - SKIP;
+            if (j >= 0 && j < NROWS) f(j, av[e] * s.w00 + bv[e] * s.w01 + cv[e] * s.w10 + dv[e] * s.w11);
+        }
+        ET_CHUNK_FENCE();
+    }
+}
+
+// Row (x) pass through LDS for a block whose thread tx owns column x' = tx:
+// out[i] = sum_j buf[tx + j][i] * kr[2R-j].  buf holds (nthr + 2R) columns of
+// TS+4 floats with R zero columns on each side.  One barrier; the caller
+// alternates two buffers so the next write cannot race these reads.
+template <int TS, int R>
+__device__ __forceinline__ void row_pass(float *buf, int tx, const float *yc, const float *__restrict__ kr,
+                                         float *out)
+{
+    constexpr int L = TS + 4;
+    float4 *w4 = reinterpret_cast<float4 *>(buf + (tx + R) * L);
+#pragma unroll
+    for (int q = 0; q < TS / 4; q++)
+        w4[q] = make_float4(yc[4 * q], yc[4 * q + 1], yc[4 * q + 2], yc[4 * q + 3]);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < TS; i++) out[i] = 0.f;
+#pragma unroll
+    for (int j = 0; j <= 2 * R; j++) {
+        const float kj = kr[2 * R - j];
+        const float4 *r4 = reinterpret_cast<const float4 *>(buf + (tx + j) * L);
+#pragma unroll
+        for (int q = 0; q < TS / 4; q++) {
+            const float4 t = r4[q];
+            out[4 * q + 0] = fmaf(t.x, kj, out[4 * q + 0]);
+            out[4 * q + 1] = fmaf(t.y, kj, out[4 * q + 1]);
+            out[4 * q + 2] = fmaf(t.z, kj, out[4 * q + 2]);
+            out[4 * q + 3] = fmaf(t.w, kj, out[4 * q + 3]);
+        }
+    }
+}
+
+struct FastView {
+    float m00, m02, m03, m20, m22, m23;
+};
+
+__device__ __forceinline__ FastView load_fast_view(const float *rec)
+{
+    FastView f;
+    f.m00 = rec[0];
+    f.m02 = rec[2];
+    f.m03 = rec[3];
+    f.m20 = rec[8];
+    f.m22 = rec[10];
+    f.m23 = rec[11];
+    return f;
+}
+
+__device__ __forceinline__ Samp fast_sample(const FastView &fv, float fxp, int z, int nx, int nz, int dx, int dz)
+{
+    const float fz = (float)z;
+    const float px = pos2(fv.m00, fxp, fv.m02, fz, fv.m03);
+    const float pz = pos2(fv.m20, fxp, fv.m22, fz, fv.m23);
+    return make_samp(px, pz, nx, nz, dx, dz);
+}
+
+// Input projection value s(v, y, x) for the backprojection sweeps.
+__device__ __forceinline__ float load_source(const EtSweepArgs &a, int v, int y, int x)
+{
+    const size_t P = (size_t)a.nx * a.ny;
+    const size_t r = (size_t)y * a.nx + x;
+    if (a.src == ET_SRC_ONES) return 1.f;
+    if (a.src == ET_SRC_SINO) return a.sino[(size_t)v * P + r];
+    // ET_SRC_RATIO: reference _et_array_interface.cpp:1098-1108 (projection
+    // truncated at 0, r = p < eps ? y/eps : y/p).
+    float p = 0.f;
+    for (int d = 0; d < a.rD; d++) p += a.rpart[((size_t)d * a.V + v) * P + r];
+    if (p < 0.f) p = 0.f;
+    const float m = a.meas[(size_t)v * P + r];
+    return (p < a.eps) ? m / a.eps : m / p;
+}
+
+// Stores TS consecutive rows of column (x', z') of view v in the U layout.
+template <int TS>
+__device__ __forceinline__ void store_u(const EtSweepArgs &a, int v, int tx, int z, int y0, const float *val)
+{
+    float *dst = a.u + (size_t)v * a.ustride + (size_t)a.yu * ((tx + 1) + (size_t)a.xp * (z + 1)) + y0;
+    float4 *d4 = reinterpret_cast<float4 *>(dst);
+#pragma unroll
+    for (int q = 0; q < TS / 4; q++) d4[q] = make_float4(val[4 * q], val[4 * q + 1], val[4 * q + 2], val[4 * q + 3]);
+}
+
+// Common geometry of a sweep block.
+struct SweepCtx {
+    int tx, nthr, v, d, y0, zb, ze, dx, dz, bufsz;
+    bool xin;
+    size_t P;
+    float fxp;
+    FastView fv;
+};
+
+template <int TS, int R>
+__device__ __forceinline__ SweepCtx sweep_ctx(const EtSweepArgs &a, bool chunked)
+{
+    SweepCtx c;
+    c.tx = threadIdx.x;
+    c.nthr = blockDim.x;
+    const int vd = blockIdx.y;
+    c.v = chunked ? vd / a.D : vd;
+    c.d = chunked ? vd - c.v * a.D : 0;
+    c.y0 = blockIdx.x * TS;
+    c.zb = chunked ? c.d * a.zlen : 0;
+    c.ze = chunked ? min(a.nz, c.zb + a.zlen) : a.nz;
+    c.xin = c.tx < a.nx;
+    c.P = (size_t)a.nx * a.ny;
+    c.bufsz = (c.nthr + 2 * R) * (TS + 4);
+    c.fv = load_fast_view(a.views + (size_t)c.v * ET_VIEW_REC);
+    c.dx = a.yp;
+    c.dz = a.yp * a.xp;
+    c.fxp = (float)c.tx;
+    return c;
+}
+
+template <int R>
+__device__ __forceinline__ void zero_lds(float *lds, int n, int tx, int nthr)
+{
+    if constexpr (R > 0) {
+        for (int i = tx; i < n; i += nthr) lds[i] = 0.f;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------
+// Sweep kernels (fast path)
+// ------------------------------------------------------------------------------
+
+// grid: (slabs, V*D); block: nthr >= nx threads, thread tx owns column x' = tx.
+// PROJECT: partial projection of chunk d written to part[d][v][y][x]
+// (the sinogram itself when D == 1).  Reference emission-lib/_et.cpp:218-281.
+template <int TS, int R, bool HAS_MU>
+__global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
+{
+    constexpr int K = 2 * R + 1;
+    constexpr int R4 = (R + 3) & ~3;
+    constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
+    extern __shared__ float lds[];
+    const SweepCtx c = sweep_ctx<TS, R>(a, true);
+    zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
+
+    float A[TS], acc[TS];
+#pragma unroll
+    for (int i = 0; i < TS; i++) {
+        A[i] = 0.f;
+        acc[i] = 0.f;
+    }
+    if (HAS_MU && c.d > 0 && c.xin) {
+        const float *as = a.astart + ((size_t)c.v * a.D + c.d) * c.P;
+#pragma unroll
+        for (int i = 0; i < TS; i++) {
+            const int y = c.y0 + i;
+            A[i] = (y < a.ny) ? as[(size_t)y * a.nx + c.tx] : 0.f;
+        }
+    }
+
+    for (int z = c.zb; z < c.ze; z++) {
+        const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
+        const float *kz = a.psf + (size_t)z * 2 * K;
+        float h[TS];
+        {
+            float yc[TS];
+            resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
+                                               [](int, float v) { return v; }, yc);
+            if constexpr (R == 0) {
+#pragma unroll
+                for (int i = 0; i < TS; i++) h[i] = yc[i];
+            } else {
+                if (!c.xin) {
+#pragma unroll
+                    for (int i = 0; i < TS; i++) yc[i] = 0.f;
+                }
+                row_pass<TS, R>(lds + (z & 1) * c.bufsz, c.tx, yc, kz, h);
+            }
+        }
+        if constexpr (HAS_MU) {
+            bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, [&](int i, float m) {
+                A[i] += m;
+                acc[i] = fmaf(h[i], exp_neg(A[i]), acc[i]);
+            });
+        } else {
+#pragma unroll
+            for (int i = 0; i < TS; i++) acc[i] += h[i];
+        }
+    }
+    if (c.d == a.D - 1) {
+#pragma unroll
+        for (int i = 0; i < TS; i++) acc[i] += HAS_MU ? a.bg_line * exp_neg(A[i]) : a.bg_line;
+    }
+    if (c.xin) {
+        float *out = a.part + ((size_t)c.d * a.V + c.v) * c.P;
+#pragma unroll
+        for (int i = 0; i < TS; i++) {
+            const int y = c.y0 + i;
+            if (y < a.ny) {
+                float val = acc[i];
+                if (a.truncate && val < 0.f) val = 0.f;
+                out[(size_t)y * a.nx + c.tx] = val;
+            }
+        }
+    }
+}
+
+// BPU: u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ) for the slab rows,
+// reference emission-lib/_et.cpp:441-471 (line backprojection, then PSF).
+template <int TS, int R, bool HAS_MU>
+__global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
+{
+    constexpr int K = 2 * R + 1;
+    constexpr int R4 = (R + 3) & ~3;
+    constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
+    extern __shared__ float lds[];
+    const SweepCtx c = sweep_ctx<TS, R>(a, true);
+    zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
+
+    float sv[NA], A[NA];
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+        const int y = c.y0 - R + j;
+        const bool ok = c.xin && y >= 0 && y < a.ny;
+        sv[j] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
+        A[j] = (HAS_MU && ok && c.d > 0) ? a.astart[((size_t)c.v * a.D + c.d) * c.P + (size_t)y * a.nx + c.tx] : 0.f;
+    }
+
+    for (int z = c.zb; z < c.ze; z++) {
+        const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
+        const float *kz = a.psf + (size_t)z * 2 * K;
+        float yc[TS];
+        if constexpr (HAS_MU) {
+            resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
+                                               [&](int j, float m) {
+                                                   A[j] += m;
+                                                   return sv[j] * exp_neg(A[j]);
+                                               },
+                                               yc);
+        } else {
+#pragma unroll
+            for (int i = 0; i < TS; i++) {
+                float t = 0.f;
+#pragma unroll
+                for (int j = 0; j <= 2 * R; j++) t = fmaf(sv[i + j], kz[K + 2 * R - j], t);
+                yc[i] = t;
+            }
+        }
+        float out[TS];
+        if constexpr (R == 0) {
+#pragma unroll
+            for (int i = 0; i < TS; i++) out[i] = yc[i];
+        } else {
+            row_pass<TS, R>(lds + (z & 1) * c.bufsz, c.tx, yc, kz, out);
+        }
+        if (c.xin) store_u<TS>(a, c.v, c.tx, z, c.y0, out);
+    }
+}
+
+// Reverse sweeps for the attenuation gradients (one block per view and slab,
+// all of z').  A(z') = A_tot - sum_{k>z'} mu'(k), A_tot precomputed per ray.
+// GRAD_REF: u = PSF( s * sum_{k>=z'} act'(k) e^{-A(k)} ), the reference kernel
+//   emission-lib_gpu/_et_attenuation_gradient_gpu_kernels.cu:15-42 followed by the
+//   PSF of emission-lib/_et.cpp:1829-1843.
+template <int TS, int R>
+__global__ void __launch_bounds__(512) k_grad_ref(EtSweepArgs a)
+{
+    constexpr int K = 2 * R + 1;
+    constexpr int R4 = (R + 3) & ~3;
+    constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
+    extern __shared__ float lds[];
+    const SweepCtx c = sweep_ctx<TS, R>(a, false);
+    zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
+    const int ys = c.y0 - R;
+
+    float sv[NA], At[NA], suf[NA], tail[NA];
+#pragma unroll
+    for (int j = 0; j < NA; j++) {
+        const int y = ys + j;
+        const bool ok = c.xin && y >= 0 && y < a.ny;
+        sv[j] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
+        At[j] = ok ? a.atot[(size_t)c.v * c.P + (size_t)y * a.nx + c.tx] : 0.f;
+        suf[j] = 0.f;
+        tail[j] = 0.f;
+    }
+    for (int z = a.nz - 1; z >= 0; z--) {
+        const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
+        const float *kz = a.psf + (size_t)z * 2 * K;
+        // mu rows first: tail needs A(z') before this plane's mu joins the suffix
+        float Az[NA];
+        bilinear_map<NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, [&](int j, float m) {
+            Az[j] = At[j] - suf[j];
+            suf[j] += m;
+        });
+        float yc[TS];
+        resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
+                                           [&](int j, float v) {
+                                               tail[j] = fmaf(v, exp_neg(Az[j]), tail[j]);
+                                               return sv[j] * tail[j];
+                                           },
+                                           yc);
+        float out[TS];
+        if constexpr (R == 0) {
+#pragma unroll
+            for (int i = 0; i < TS; i++) out[i] = yc[i];
+        } else {
+            row_pass<TS, R>(lds + (z & 1) * c.bufsz, c.tx, yc, kz, out);
+        }
+        if (c.xin) store_u<TS>(a, c.v, c.tx, z, c.y0, out);
+    }
+}
+
+// GRAD_MU: u = -s * ( sum_{k>=z'} PSF(act')(k) e^{-A(k)} + bg e^{-A_tot} ), the
+// exact derivative of the projection with respect to mu'(z') (not in the reference).
+template <int TS, int R>
+__global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
+{
+    constexpr int K = 2 * R + 1;
+    constexpr int R4 = (R + 3) & ~3;
+    constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
+    extern __shared__ float lds[];
+    const SweepCtx c = sweep_ctx<TS, R>(a, false);
+    zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
+
+    float sv[TS], At[TS], suf[TS], tail[TS];
+#pragma unroll
+    for (int i = 0; i < TS; i++) {
+        const int y = c.y0 + i;
+        const bool ok = c.xin && y < a.ny;
+        sv[i] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
+        At[i] = ok ? a.atot[(size_t)c.v * c.P + (size_t)y * a.nx + c.tx] : 0.f;
+        suf[i] = 0.f;
+        tail[i] = a.bg_line * exp_neg(At[i]);
+    }
+    for (int z = a.nz - 1; z >= 0; z--) {
+        const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
+        const float *kz = a.psf + (size_t)z * 2 * K;
+        float h[TS];
+        {
+            float yc[TS];
+            resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
+                                               [](int, float v) { return v; }, yc);
+            if constexpr (R == 0) {
+#pragma unroll
+                for (int i = 0; i < TS; i++) h[i] = yc[i];
+            } else {
+                if (!c.xin) {
+#pragma unroll
+                    for (int i = 0; i < TS; i++) yc[i] = 0.f;
+                }
+                row_pass<TS, R>(lds + (z & 1) * c.bufsz, c.tx, yc, kz, h);
+            }
+        }
+        float out[TS];
+        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, [&](int i, float m) {
+            const float Az = At[i] - suf[i];
+            suf[i] += m;
+            tail[i] = fmaf(h[i], exp_neg(Az), tail[i]);
+            out[i] = -sv[i] * tail[i];
+        });
+        if (c.xin) store_u<TS>(a, c.v, c.tx, z, c.y0, out);
+    }
+}
+
+// MUSUM: part[v][d][y][x] = sum over chunk d of mu'(x, y, z').
+template <int TS>
+__global__ void __launch_bounds__(512) k_musum(EtSweepArgs a)
+{
+    const SweepCtx c = sweep_ctx<TS, 0>(a, true);
+    if (!c.xin) return;
+    float acc[TS];
+#pragma unroll
+    for (int i = 0; i < TS; i++) acc[i] = 0.f;
+    for (int z = c.zb; z < c.ze; z++) {
+        const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
+        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, [&](int i, float m) { acc[i] += m; });
+    }
+    float *out = a.part + ((size_t)c.v * a.D + c.d) * c.P;
+#pragma unroll
+    for (int i = 0; i < TS; i++) {
+        const int y = c.y0 + i;
+        if (y < a.ny) out[(size_t)y * a.nx + c.tx] = acc[i];
+    }
+}
+
+// ------------------------------------------------------------------------------
+// Pull kernels (fast path): accumulate the rotated-frame volumes of V views into
+// the output grid.  Thread = output column (x, z) and a run of T axial rows.
+// ------------------------------------------------------------------------------
+
+// Reference pull: inverse rotation (negated angles, ZYX order), trilinear
+// pull, accumulation in view order (emission-lib/_et.cpp:474-498).  The fast
+// path runs with zero padding only; a non-zero background takes the general path.
+__device__ __forceinline__ void pull_ref_view(const EtPullArgs &a, const float *rec, const float *uv,
+                                              int x, int z, int y0, float *acc)
+{
+    const float qx = pos2(rec[12], (float)x, rec[14], (float)z, rec[15]);
+    const float qz = pos2(rec[20], (float)x, rec[22], (float)z, rec[23]);
+    const int dx = a.yu, dz = a.yu * a.xp;
+    const Samp s = make_samp(qx, qz, a.nx, a.nz, dx, dz);
+    bilinear_map<T, T, 0>(uv + s.off + y0, dx, dz, s, [&](int i, float v) { acc[i] += v; });
+}
+
+// Exact adjoint of the forward fast-path pull: gathers from every rotated-frame
+// column (x', z') whose forward sample has (x, z) among its corners, with the
+// forward weight.  Candidates are bounded through the inverse linear map;
+// membership and weights are recomputed with the forward arithmetic, so this is
+// the transpose of exactly what k_project / k_grad sample.
+__device__ __forceinline__ void pull_adj_view(const EtPullArgs &a, const float *rec, const float *uv,
+                                              int x, int z, int y0, float *acc)
+{
+    const float f00 = rec[0], f02 = rec[2], f03 = rec[3], f20 = rec[8], f22 = rec[10], f23 = rec[11];
+    const float i00 = rec[24], i02 = rec[26], i20 = rec[30], i22 = rec[32];
+    const float rx = (float)x - f03, rz = (float)z - f23;
+    const float cx = i00 * rx + i02 * rz, cz = i20 * rx + i22 * rz;
+    const float hx = fabsf(i00) + fabsf(i02) + 0.01f, hz = fabsf(i20) + fabsf(i22) + 0.01f;
+    const int xlo = max(0, (int)ceilf(cx - hx)), xhi = min(a.nx - 1, (int)floorf(cx + hx));
+    const int zlo = max(0, (int)ceilf(cz - hz)), zhi = min(a.nz - 1, (int)floorf(cz + hz));
+    const float fxo = (float)x, fzo = (float)z;
+    for (int xc = xlo; xc <= xhi; xc++) {
+        for (int zc = zlo; zc <= zhi; zc++) {
+            const float px = pos2(f00, (float)xc, f02, (float)zc, f03);
+            const float pz = pos2(f20, (float)xc, f22, (float)zc, f23);
+            const float fx = floorf(px), fz = floorf(pz);
+            const float rxx = px - fx, rzz = pz - fz;
+            float wx = (fx == fxo) ? 1.f - rxx : ((fx == fxo - 1.f) ? rxx : 0.f);
+            float wz = (fz == fzo) ? 1.f - rzz : ((fz == fzo - 1.f) ? rzz : 0.f);
+            const float w = wx * wz;
+            if (w != 0.f) {
+                const float4 *q = reinterpret_cast<const float4 *>(
+                    uv + (size_t)a.yu * ((xc + 1) + (size_t)a.xp * (zc + 1)) + y0);
+#pragma unroll
+                for (int k = 0; k < T / 4; k++) {
+                    float4 t = q[k];
+                    acc[4 * k + 0] = fmaf(t.x, w, acc[4 * k + 0]);
+                    acc[4 * k + 1] = fmaf(t.y, w, acc[4 * k + 1]);
+                    acc[4 * k + 2] = fmaf(t.z, w, acc[4 * k + 2]);
+                    acc[4 * k + 3] = fmaf(t.w, w, acc[4 * k + 3]);
+                }
+            }
+        }
+    }
+}
+
+template <bool ADJ, int EPI>
+__global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nx * a.nz) return;
+    const int x = c % a.nx, z = c / a.nx;
+    const int y0 = blockIdx.y * T;
+    float acc[T];
+#pragma unroll
+    for (int i = 0; i < T; i++) acc[i] = 0.f;
+    for (int v = 0; v < a.V; v++) {
+        const float *rec = a.views + (size_t)v * ET_VIEW_REC;
+        const float *uv = a.u + (size_t)v * a.ustride;
+        if (ADJ)
+            pull_adj_view(a, rec, uv, x, z, y0, acc);
+        else
+            pull_ref_view(a, rec, uv, x, z, y0, acc);
+    }
+    if constexpr (EPI == ET_EPI_ABI_STORE) {
+#pragma unroll
+        for (int i = 0; i < T; i++) {
+            const int y = y0 + i;
+            if (y < a.ny) {
+                float *o = a.out + x + (size_t)a.nx * (y + (size_t)a.ny * z);
+                float val = a.accumulate ? *o + acc[i] : acc[i];
+                if (a.truncate && val < 0.f) val = 0.f;
+                *o = val;
+            }
+        }
+    } else if constexpr (EPI == ET_EPI_YF_STORE) {
+        float *o = a.out + (a.gy + y0) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
+#pragma unroll
+        for (int i = 0; i < T; i++) {
+            const int y = y0 + i;
+            if (y < a.ny) {
+                float val = a.accumulate ? o[i] + acc[i] : acc[i];
+                if (a.truncate && val < 0.f) val = 0.f;
+                o[i] = val;
+            }
+        }
+    } else {
+        const size_t base = (a.gy + y0) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
+#pragma unroll
+        for (int i = 0; i < T; i++) {
+            const int y = y0 + i;
+            if (y < a.ny) {
+                float b = acc[i] < 0.f ? 0.f : acc[i];
+                a.x[base + i] = a.x[base + i] * (b / a.sens[base + i]);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
+// General path
+// ------------------------------------------------------------------------------
+
+struct AbiAccess {
+    const float *p;
+    int nx, ny;
+    __device__ float operator()(int X, int Y, int Z) const
+    {
+        return p[X + (size_t)nx * (Y + (size_t)ny * Z)];
+    }
+};
+struct UAccess {
+    const float *p;
+    int yu, xp;
+    __device__ float operator()(int X, int Y, int Z) const
+    {
+        return p[Y + (size_t)yu * ((X + 1) + (size_t)xp * (Z + 1))];
+    }
+};
+
+// Trilinear pull with the reference's loop structure, borders and background
+// rule, reg-lib/_reg_resampling.cpp:305-450.
+template <class Acc>
+__device__ float trilinear_ref(const Acc &src, int nx, int ny, int nz, float px, float py, float pz,
+                               float bg)
+{
+    float p[3] = {px, py, pz};
+    float pv[3], b0[3], b1[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        pv[i] = floorf(p[i]);
+        float rel = p[i] - pv[i];
+        if (rel < 0.f) rel = 0.f;
+        b0[i] = 1.f - rel;
+        b1[i] = rel;
+    }
+    const float n[3] = {(float)nx, (float)ny, (float)nz};
+    float intensity = 0.f, bgw = 0.f;
+    for (int c = 0; c < 2; c++) {
+        const float Z = pv[2] + c;
+        if (Z >= 0.f && Z < n[2]) {
+            float yt = 0.f;
+            for (int b = 0; b < 2; b++) {
+                const float Y = pv[1] + b;
+                if (Y >= 0.f && Y < n[1]) {
+                    float xt = 0.f;
+                    for (int aa = 0; aa < 2; aa++) {
+                        const float X = pv[0] + aa;
+                        if (X >= 0.f && X < n[0])
+                            xt += src((int)X, (int)Y, (int)Z) * (aa ? b1[0] : b0[0]);
+                        else if (bg != 0.f)
+                            bgw += bg * (aa ? b1[0] : b0[0]);
+                    }
+                    yt += xt * (b ? b1[1] : b0[1]);
+                } else if (bg != 0.f)
+                    bgw += bg * (b ? b1[0] : b0[0]);
+            }
+            intensity += yt * (c ? b1[2] : b0[2]);
+        } else if (bg != 0.f)
+            bgw += bg * (c ? b1[0] : b0[0]);
+    }
+    if (bg != 0.f) intensity += bgw;
+    return intensity;
+}
+
+// dst(v) = src(M v) for every voxel v of the grid; dst in YF (dst_yf) or ABI layout.
+__global__ void __launch_bounds__(256) k_resample_general(const float *__restrict__ src, int nx,
+                                                          int ny, int nz, const float *__restrict__ M,
+                                                          float bg, float *__restrict__ dst, int dst_yf,
+                                                          int yp, int gy)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (i >= n) return;
+    int x, y, z;
+    if (dst_yf) {  // thread order follows the destination: y fastest
+        y = (int)(i % ny);
+        size_t r = i / ny;
+        x = (int)(r % nx);
+        z = (int)(r / nx);
+    } else {
+        x = (int)(i % nx);
+        size_t r = i / nx;
+        y = (int)(r % ny);
+        z = (int)(r / ny);
+    }
+    const float fx = (float)x, fy = (float)y, fz = (float)z;
+    const float px = pos3(M, fx, fy, fz), py = pos3(M + 4, fx, fy, fz), pz = pos3(M + 8, fx, fy, fz);
+    AbiAccess acc{src, nx, ny};
+    const float val = trilinear_ref(acc, nx, ny, nz, px, py, pz, bg);
+    if (dst_yf)
+        dst[(gy + y) + (size_t)yp * ((x + 1) + (size_t)(nx + 2) * (z + 1))] = val;
+    else
+        dst[x + (size_t)nx * (y + (size_t)ny * z)] = val;
+}
+
+// General pull of one view into the output (accumulating in place).
+// Reference mode: inverse matrix (record[12..23]) with the reference borders.
+// Adjoint mode: transpose of k_resample_general's pull with the forward matrix.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_pull_general(EtPullArgs a)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)a.nx * a.ny * a.nz;
+    if (i >= n) return;
+    const int x = (int)(i % a.nx);
+    const size_t r = i / a.nx;
+    const int y = (int)(r % a.ny);
+    const int z = (int)(r / a.ny);
+    const float *rec = a.views;
+    const float *uv = a.u;
+    float val = 0.f;
+    if (!a.adjoint) {
+        const float *Mi = rec + 12;
+        const float fx = (float)x, fy = (float)y, fz = (float)z;
+        UAccess acc{uv, a.yu, a.xp};
+        val = trilinear_ref(acc, a.nx, a.ny, a.nz, pos3(Mi, fx, fy, fz), pos3(Mi + 4, fx, fy, fz),
+                            pos3(Mi + 8, fx, fy, fz), a.bg);
+    } else {
+        const float *F = rec;
+        const float *Fi = rec + 24;
+        const float rv[3] = {(float)x - F[3], (float)y - F[7], (float)z - F[11]};
+        int lo[3], hi[3];
+        const int nn[3] = {a.nx, a.ny, a.nz};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float c = Fi[3 * k] * rv[0] + Fi[3 * k + 1] * rv[1] + Fi[3 * k + 2] * rv[2];
+            const float h = fabsf(Fi[3 * k]) + fabsf(Fi[3 * k + 1]) + fabsf(Fi[3 * k + 2]) + 0.01f;
+            lo[k] = max(0, (int)ceilf(c - h));
+            hi[k] = min(nn[k] - 1, (int)floorf(c + h));
+        }
+        const float o[3] = {(float)x, (float)y, (float)z};
+        for (int zc = lo[2]; zc <= hi[2]; zc++)
+            for (int yc = lo[1]; yc <= hi[1]; yc++)
+                for (int xc = lo[0]; xc <= hi[0]; xc++) {
+                    const float p[3] = {pos3(F, (float)xc, (float)yc, (float)zc),
+                                        pos3(F + 4, (float)xc, (float)yc, (float)zc),
+                                        pos3(F + 8, (float)xc, (float)yc, (float)zc)};
+                    float w = 1.f;
+#pragma unroll
+                    for (int k = 0; k < 3; k++) {
+                        const float f = floorf(p[k]), rr = p[k] - f;
+                        w *= (f == o[k]) ? 1.f - rr : ((f == o[k] - 1.f) ? rr : 0.f);
+                    }
+                    if (w != 0.f) val = fmaf(w, uv[yc + (size_t)a.yu * ((xc + 1) + (size_t)a.xp * (zc + 1))], val);
+                }
+    }
+    if constexpr (EPI == ET_EPI_ABI_STORE) {
+        float *o = a.out + i;
+        float res = a.accumulate ? *o + val : val;
+        if (a.truncate && res < 0.f) res = 0.f;
+        *o = res;
+    } else {
+        float *o = a.out + (a.gy + y) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
+        float res = a.accumulate ? *o + val : val;
+        if (a.truncate && res < 0.f) res = 0.f;
+        *o = res;
+    }
+}
+
+// One axis of the depth-dependent separable convolution with a runtime radius
+// (general path, PSF radius beyond ET_RMAX).  axis 0: row kernel along x,
+// axis 1: column kernel along y.  flip: correlation (adjoint).
+__global__ void __launch_bounds__(256) k_conv_axis(const float *__restrict__ src, float *__restrict__ dst,
+                                                   int nx, int ny, int nz, long sx, long sy, long sz,
+                                                   long off, int axis, const float *__restrict__ sep,
+                                                   int K, int flip)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (i >= n) return;
+    const int x = (int)(i % nx);
+    const size_t r = i / nx;
+    const int y = (int)(r % ny);
+    const int z = (int)(r / ny);
+    const int R = (K - 1) / 2;
+    const float *ker = sep + (size_t)z * 2 * K + (axis ? K : 0);
+    const long base = off + x * sx + y * sy + z * sz;
+    const int pos = axis ? y : x, len = axis ? ny : nx;
+    const long st = axis ? sy : sx;
+    float s = 0.f;
+    for (int k = -R; k <= R; k++) {
+        const int d = pos + k;
+        if (d >= 0 && d < len) s += src[base + (long)k * st] * ker[flip ? R + k : R - k];
+    }
+    dst[base] = s;
+}
+
+// ------------------------------------------------------------------------------
+// Layout and elementwise kernels
+// ------------------------------------------------------------------------------
+
+// ABI [z][y][x] -> YF interior [z][x][y] through a 32x32 tile per z plane.
+__global__ void __launch_bounds__(256) k_pack_yf(const float *__restrict__ abi, float *__restrict__ yf,
+                                                 int nx, int ny, int nz, int yp, int gy)
+{
+    __shared__ float tile[32][33];
+    const int z = blockIdx.z;
+    const int xb = blockIdx.x * 32, yb = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const int x = xb + tx, y = yb + k;
+        tile[k][tx] = (x < nx && y < ny) ? abi[x + (size_t)nx * (y + (size_t)ny * z)] : 0.f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int y = yb + tx, x = xb + k;
+        if (x < nx && y < ny) yf[(gy + y) + (size_t)yp * ((x + 1) + (size_t)(nx + 2) * (z + 1))] = tile[tx][k];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_unpack_yf(const float *__restrict__ yf, float *__restrict__ abi,
+                                                   int nx, int ny, int nz, int yp, int gy)
+{
+    __shared__ float tile[32][33];
+    const int z = blockIdx.z;
+    const int xb = blockIdx.x * 32, yb = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int k = ty; k < 32; k += 8) {
+        const int y = yb + tx, x = xb + k;
+        tile[k][tx] = (x < nx && y < ny) ? yf[(gy + y) + (size_t)yp * ((x + 1) + (size_t)(nx + 2) * (z + 1))] : 0.f;
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const int x = xb + tx, y = yb + k;
+        if (x < nx && y < ny) abi[x + (size_t)nx * (y + (size_t)ny * z)] = tile[tx][k];
+    }
+}
+
+// astart[v][d] = sum_{d' < d} csum[v][d'], atot[v] = sum_d csum[v][d].
+__global__ void __launch_bounds__(256) k_chunk_scan(const float *__restrict__ csum, float *__restrict__ astart,
+                                                    float *__restrict__ atot, int V, int D, int P)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)V * P) return;
+    const int v = (int)(i / P);
+    const size_t r = i % P;
+    float run = 0.f;
+    for (int d = 0; d < D; d++) {
+        const size_t k = ((size_t)v * D + d) * P + r;
+        if (astart) astart[k] = run;
+        run += csum[k];
+    }
+    if (atot) atot[i] = run;
+}
+
+__global__ void __launch_bounds__(256) k_reduce_parts(const float *__restrict__ part, int D, size_t VP,
+                                                      float *__restrict__ sino, int truncate)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= VP) return;
+    float s = 0.f;
+    for (int d = 0; d < D; d++) s += part[(size_t)d * VP + i];
+    if (truncate && s < 0.f) s = 0.f;
+    sino[i] = s;
+}
+
+__global__ void __launch_bounds__(256) k_bg_only(const float *__restrict__ atot, float bg, size_t n,
+                                                 float *__restrict__ sino, int truncate)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = bg * exp_neg(atot[i]);
+    if (truncate && s < 0.f) s = 0.f;
+    sino[i] = s;
+}
+
+__global__ void __launch_bounds__(256) k_fill(float *buf, size_t n, float value)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) buf[i] = value;
+}
+
+__global__ void __launch_bounds__(256) k_clamp_min(float *buf, size_t n, float lo)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && buf[i] < lo) buf[i] = lo;
+}
+
+__global__ void __launch_bounds__(256) k_ratio(const float *__restrict__ meas, const float *__restrict__ proj,
+                                               float *__restrict__ out, size_t n, float eps)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float p = proj[i];
+    out[i] = (p < eps) ? meas[i] / eps : meas[i] / p;
+}
+
+// YF interior only: f(i) for each logical voxel.
+template <class F>
+__global__ void __launch_bounds__(256) k_yf_map(int nx, int ny, int nz, int yp, int gy, F f)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t n = (size_t)nx * ny * nz;
+    if (i >= n) return;
+    const int y = (int)(i % ny);
+    const size_t r = i / ny;
+    const int x = (int)(r % nx);
+    const int z = (int)(r / nx);
+    f((gy + y) + (size_t)yp * ((x + 1) + (size_t)(nx + 2) * (z + 1)));
+}
+
+struct ClampMinOp {
+    float *p;
+    float lo;
+    __device__ void operator()(size_t k) const
+    {
+        if (p[k] < lo) p[k] = lo;
+    }
+};
+struct OsemOp {
+    float *x;
+    const float *b, *s;
+    __device__ void operator()(size_t k) const
+    {
+        float bb = b[k] < 0.f ? 0.f : b[k];
+        x[k] = x[k] * (bb / s[k]);
+    }
+};
+
+__global__ void __launch_bounds__(256) k_em_update_flat(float *__restrict__ x, const float *__restrict__ b,
+                                                        const float *__restrict__ s, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float bb = b[i] < 0.f ? 0.f : b[i];
+    x[i] = x[i] * (bb / s[i]);
+}
+
+inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------
+
+// Slab height (axial rows per block) of each sweep kernel: taller slabs
+// amortise the PSF halo, shorter ones keep the register footprint down.
+static int slab_rows(int mode, int R)
+{
+    switch (mode) {
+    case ET_MODE_PROJECT: return 16;
+    case ET_MODE_BPU: return R <= 7 ? 16 : 8;
+    case ET_MODE_GRAD_REF: return R <= 2 ? 8 : 4;
+    case ET_MODE_GRAD_MU: return R <= 6 ? 16 : 8;
+    default: return 16;
+    }
+}
+
+int et_sweep_slab_rows(int mode, int R) { return slab_rows(mode, R); }
+
+static size_t sweep_lds_bytes(int R, int TS, int nthr)
+{
+    return R > 0 ? (size_t)2 * (nthr + 2 * R) * (TS + 4) * sizeof(float) : 0;
+}
+
+template <int TS, int R>
+static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st)
+{
+    const int nthr = ((a.nx + 63) / 64) * 64;
+    const size_t lds = sweep_lds_bytes(R, TS, nthr);
+    const int slabs = (a.ny + TS - 1) / TS;
+    if (mode == ET_MODE_PROJECT) {
+        dim3 g(slabs, a.V * a.D);
+        if (has_mu)
+            hipLaunchKernelGGL((k_project<TS, R, true>), g, dim3(nthr), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_project<TS, R, false>), g, dim3(nthr), lds, st, a);
+    } else if (mode == ET_MODE_BPU) {
+        dim3 g(slabs, a.V * a.D);
+        if (has_mu)
+            hipLaunchKernelGGL((k_bpu<TS, R, true>), g, dim3(nthr), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_bpu<TS, R, false>), g, dim3(nthr), lds, st, a);
+    } else if (mode == ET_MODE_GRAD_REF) {
+        hipLaunchKernelGGL((k_grad_ref<TS, R>), dim3(slabs, a.V), dim3(nthr), lds, st, a);
+    } else if (mode == ET_MODE_GRAD_MU) {
+        hipLaunchKernelGGL((k_grad_mu<TS, R>), dim3(slabs, a.V), dim3(nthr), lds, st, a);
+    } else {
+        return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+template <int R>
+static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st)
+{
+    switch (slab_rows(mode, R)) {
+    case 16: return launch_sweep_ts<16, R>(mode, has_mu, a, st);
+    case 8: return launch_sweep_ts<8, R>(mode, has_mu, a, st);
+    default: return launch_sweep_ts<4, R>(mode, has_mu, a, st);
+    }
+}
+
+int et_launch_sweep(int mode, int R, int has_mu, int has_act, const EtSweepArgs &a, et_stream_t stream)
+{
+    (void)has_act;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (mode == ET_MODE_MUSUM) {
+        const int nthr = ((a.nx + 63) / 64) * 64;
+        hipLaunchKernelGGL(k_musum<16>, dim3((a.ny + 15) / 16, a.V * a.D), dim3(nthr), 0, st, a);
+        return (int)hipGetLastError();
+    }
+    switch (R) {
+#define ET_R_CASE(r) \
+    case r: return launch_sweep_r<r>(mode, has_mu, a, st);
+        ET_R_CASE(0) ET_R_CASE(1) ET_R_CASE(2) ET_R_CASE(3) ET_R_CASE(4) ET_R_CASE(5)
+        ET_R_CASE(6) ET_R_CASE(7) ET_R_CASE(8) ET_R_CASE(9) ET_R_CASE(10)
+#undef ET_R_CASE
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+int et_sweep_variant_count(void) { return (ET_RMAX + 1) * 6 + 1; }
+
+int et_launch_pull(const EtPullArgs &a, et_stream_t stream)
+{
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    dim3 g(nblk((size_t)a.nx * a.nz, 256), (a.ny + T - 1) / T);
+#define ET_PULL(adj, epi) hipLaunchKernelGGL((k_pull<adj, epi>), g, dim3(256), 0, st, a)
+    if (a.adjoint) {
+        if (a.epilogue == ET_EPI_ABI_STORE) ET_PULL(true, ET_EPI_ABI_STORE);
+        else if (a.epilogue == ET_EPI_YF_STORE) ET_PULL(true, ET_EPI_YF_STORE);
+        else ET_PULL(true, ET_EPI_OSEM);
+    } else {
+        if (a.epilogue == ET_EPI_ABI_STORE) ET_PULL(false, ET_EPI_ABI_STORE);
+        else if (a.epilogue == ET_EPI_YF_STORE) ET_PULL(false, ET_EPI_YF_STORE);
+        else ET_PULL(false, ET_EPI_OSEM);
+    }
+#undef ET_PULL
+    return (int)hipGetLastError();
+}
+
+int et_launch_pull_general(const EtPullArgs &a, et_stream_t stream)
+{
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const size_t n = (size_t)a.nx * a.ny * a.nz;
+    if (a.epilogue == ET_EPI_ABI_STORE)
+        hipLaunchKernelGGL((k_pull_general<ET_EPI_ABI_STORE>), dim3(nblk(n, 256)), dim3(256), 0, st, a);
+    else if (a.epilogue == ET_EPI_YF_STORE)
+        hipLaunchKernelGGL((k_pull_general<ET_EPI_YF_STORE>), dim3(nblk(n, 256)), dim3(256), 0, st, a);
+    else
+        return (int)hipErrorInvalidValue;
+    return (int)hipGetLastError();
+}
+
+int et_launch_pack_yf(const float *abi, float *yf, int nx, int ny, int nz, int yp, int gy, et_stream_t stream)
+{
+    dim3 g((nx + 31) / 32, (ny + 31) / 32, nz);
+    hipLaunchKernelGGL(k_pack_yf, g, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), abi, yf, nx, ny, nz, yp, gy);
+    return (int)hipGetLastError();
+}
+
+int et_launch_unpack_yf(const float *yf, float *abi, int nx, int ny, int nz, int yp, int gy, et_stream_t stream)
+{
+    dim3 g((nx + 31) / 32, (ny + 31) / 32, nz);
+    hipLaunchKernelGGL(k_unpack_yf, g, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), yf, abi, nx, ny, nz, yp, gy);
+    return (int)hipGetLastError();
+}
+
+int et_launch_resample_general(const float *src, int nx, int ny, int nz, const float *M12, float bg, float *dst,
+                               int dst_yf, int yp, int gy, et_stream_t stream)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_resample_general, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       src, nx, ny, nz, M12, bg, dst, dst_yf, yp, gy);
+    return (int)hipGetLastError();
+}
+
+int et_launch_chunk_scan(const float *csum, float *astart, float *atot, int V, int D, int P, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_chunk_scan, dim3(nblk((size_t)V * P, 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), csum, astart, atot, V, D, P);
+    return (int)hipGetLastError();
+}
+
+int et_launch_reduce_parts(const float *part, int D, int V, int P, float *sino, int truncate, et_stream_t stream)
+{
+    const size_t VP = (size_t)V * P;
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nblk(VP, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       part, D, VP, sino, truncate);
+    return (int)hipGetLastError();
+}
+
+int et_launch_bg_only(const float *atot, float bg, int n, float *sino, int truncate, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_bg_only, dim3(nblk((size_t)n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       atot, bg, (size_t)n, sino, truncate);
+    return (int)hipGetLastError();
+}
+
+int et_launch_conv_axis(const float *src, float *dst, int nx, int ny, int nz, long sx, long sy, long sz, long off,
+                        int axis, const float *sep, int K, int flip, et_stream_t stream)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_conv_axis, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), src,
+                       dst, nx, ny, nz, sx, sy, sz, off, axis, sep, K, flip);
+    return (int)hipGetLastError();
+}
+
+int et_launch_truncate(float *buf, size_t n, et_stream_t stream) { return et_launch_clamp_min(buf, n, 0.f, stream); }
+
+int et_launch_fill(float *buf, size_t n, float value, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), buf, n, value);
+    return (int)hipGetLastError();
+}
+
+int et_launch_clamp_min(float *buf, size_t n, float lo, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_clamp_min, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), buf, n, lo);
+    return (int)hipGetLastError();
+}
+
+int et_launch_yf_clamp_min(float *yf, int nx, int ny, int nz, int yp, int gy, float lo, et_stream_t stream)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((k_yf_map<ClampMinOp>), dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       nx, ny, nz, yp, gy, ClampMinOp{yf, lo});
+    return (int)hipGetLastError();
+}
+
+int et_launch_osem_yf_update(float *x_yf, const float *b_yf, const float *s_yf, int nx, int ny, int nz, int yp, int gy,
+                             et_stream_t stream)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    hipLaunchKernelGGL((k_yf_map<OsemOp>), dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       nx, ny, nz, yp, gy, OsemOp{x_yf, b_yf, s_yf});
+    return (int)hipGetLastError();
+}
+
+int et_launch_ratio(const float *meas, const float *proj, float *out, size_t n, float eps, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_ratio, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), meas, proj,
+                       out, n, eps);
+    return (int)hipGetLastError();
+}
+
+int et_launch_em_update_flat(float *x, const float *b, const float *s, size_t n, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_em_update_flat, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), x, b,
+                       s, n);
+    return (int)hipGetLastError();
+}
