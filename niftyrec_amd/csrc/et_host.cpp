@@ -186,8 +186,10 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
-    // Grows to at least `count` floats; new memory is zero-filled when `zero`.
-    float *ensure(size_t count, bool zero)
+    // Grows to at least `count` floats; new memory is zero-filled when `zero`,
+    // by a kernel on `stream`, so the fill is ordered before the stream's later
+    // work whatever kind of stream the caller uses.
+    float *ensure(size_t count, bool zero, hipStream_t stream = nullptr)
     {
         if (count <= n && p) return p;
         release();
@@ -196,7 +198,7 @@ struct DevBuf {
             throw EtError{niftyrec_error_allocgpu};
         }
         n = count;
-        if (zero) check_hip(hipMemset(p, 0, count * sizeof(float)), niftyrec_error_allocgpu);
+        if (zero) check_launch(et_launch_fill(p, count, 0.f, (et_stream_t)stream));
         return p;
     }
 };
@@ -418,7 +420,7 @@ int view_batch(et_plan *p)
 
 void pack(et_plan *p, const float *abi, DevBuf &yf)
 {
-    yf.ensure(p->yf_elems, true);
+    yf.ensure(p->yf_elems, true, p->st);
     ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
     check_launch(et_launch_pack_yf(abi, yf.p, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
 }
@@ -534,7 +536,7 @@ void pull_fast(et_plan *p, const float *recs, int nv, const float *u, bool adjoi
 // radius is too large for the sweep kernels).
 void rotate_general(et_plan *p, const float *abi, int v, float bg, DevBuf &dst, bool psf, bool flip)
 {
-    dst.ensure(p->yf_elems, true);
+    dst.ensure(p->yf_elems, true, p->st);
     {
         ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
         check_launch(et_launch_resample_general(abi, p->nx, p->ny, p->nz, p->d_recs.p + (size_t)v * ET_VIEW_REC, bg,
@@ -542,7 +544,7 @@ void rotate_general(et_plan *p, const float *abi, int v, float bg, DevBuf &dst, 
     }
     if (psf && p->K > 0 && p->R > ET_RMAX) {
         // row (x) pass then column (y) pass, emission-lib/_et_convolveSeparable2D.cpp:43-75
-        p->tmp.ensure(p->yf_elems, true);
+        p->tmp.ensure(p->yf_elems, true, p->st);
         const long sx = p->yp, sy = 1, sz = (long)p->yp * p->xp, off = p->gy + (long)p->yp * (1 + p->xp);
         const float *sep = flip ? p->d_sep_flip.p : p->d_sep.p;
         check_launch(et_launch_conv_axis(dst.p, p->tmp.p, p->nx, p->ny, p->nz, sx, sy, sz, off, 0, sep, p->K, 0,
@@ -558,7 +560,7 @@ int general_R(et_plan *p) { return p->R > ET_RMAX ? 0 : p->R; }
 void conv_u_general(et_plan *p, float *u, bool flip)
 {
     if (!(p->K > 0 && p->R > ET_RMAX)) return;
-    p->tmp2.ensure(p->ustride, true);
+    p->tmp2.ensure(p->ustride, true, p->st);
     const long sx = p->yu, sy = 1, sz = (long)p->yu * p->xp, off = (long)p->yu * (1 + p->xp);
     const float *sep = flip ? p->d_sep_flip.p : p->d_sep.p;
     if (!flip) {
@@ -665,7 +667,7 @@ void op_backproject(et_plan *p, const float *sino, const float *mu, float *out, 
             mu_yf = p->mu_yf.p;
         }
         const int vb = view_batch(p);
-        p->u.ensure((size_t)vb * p->ustride, true);
+        p->u.ensure((size_t)vb * p->ustride, true, p->st);
         for (int v0 = 0; v0 < p->V; v0 += vb) {
             const int nv = std::min(vb, p->V - v0);
             const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
@@ -683,7 +685,7 @@ void op_backproject(et_plan *p, const float *sino, const float *mu, float *out, 
         return;
     }
     const int R = general_R(p);
-    p->u.ensure(p->ustride, true);
+    p->u.ensure(p->ustride, true, p->st);
     for (int v = 0; v < p->V; v++) {
         if (mu) rotate_general(p, mu, v, p->bga, p->rot_mu, false, false);
         EtSweepArgs a = base_args(p);
@@ -712,7 +714,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
         pack(p, act, p->act_yf);
         pack(p, mu, p->mu_yf);
         const int vb = view_batch(p);
-        p->u.ensure((size_t)vb * p->ustride, true);
+        p->u.ensure((size_t)vb * p->ustride, true, p->st);
         for (int v0 = 0; v0 < p->V; v0 += vb) {
             const int nv = std::min(vb, p->V - v0);
             const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
@@ -736,7 +738,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
         return;
     }
     const int R = general_R(p);
-    p->u.ensure(p->ustride, true);
+    p->u.ensure(p->ustride, true, p->st);
     for (int v = 0; v < p->V; v++) {
         rotate_general(p, mu, v, p->bga, p->rot_mu, false, false);
         rotate_general(p, act, v, p->bg, p->rot_act, exact, false);
@@ -767,7 +769,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
 void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const float *mu_yf, const float *meas,
                       const float *sens_yf, float eps, int D, const float *astart)
 {
-    p->u.ensure((size_t)nv * p->ustride, true);
+    p->u.ensure((size_t)nv * p->ustride, true, p->st);
     float *parts = p->part.ensure((size_t)D * nv * p->P, false);
     project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, D, astart, parts);
     bpu_fast(p, recs, nv, mu_yf, ET_SRC_RATIO, nullptr, meas, parts, D, eps, D, astart, false, p->u.p);
@@ -779,7 +781,7 @@ void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const 
 void em_sensitivity(et_plan *p, const float *recs, int nv, const float *mu_yf, float eps, int D,
                     const float *astart, float *sens_yf)
 {
-    p->u.ensure((size_t)nv * p->ustride, true);
+    p->u.ensure((size_t)nv * p->ustride, true, p->st);
     bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, astart, false, p->u.p);
     pull_fast(p, recs, nv, p->u.p, false, ET_EPI_YF_STORE, false, true, sens_yf);
     ProfScope ps(p, ET_PROF_OTHER, 0);
@@ -838,7 +840,7 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
         check_hip(hipMemcpyAsync(p->osem_recs.p, recs.data(), sizeof(float) * recs.size(), hipMemcpyHostToDevice,
                                  p->st),
                   niftyrec_error_transfergpu);
-        p->sens.ensure((size_t)S * p->yf_elems, true);
+        p->sens.ensure((size_t)S * p->yf_elems, true, p->st);
         if (mu_yf && D > 1) p->osem_astart.ensure((size_t)p->V * D * p->P, false);
         for (int s = 0; s < S; s++) {
             const int nv = start[s + 1] - start[s];

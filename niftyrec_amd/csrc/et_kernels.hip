@@ -202,6 +202,17 @@ struct FastView {
     float m00, m02, m03, m20, m22, m23;
 };
 
+// Separated kernels of depth plane z (row kernel then column kernel); the PSF
+// pointer is null when there is no PSF, so no address is formed for R == 0.
+template <int R>
+__device__ __forceinline__ const float *psf_plane(const float *psf, int z)
+{
+    if constexpr (R == 0)
+        return nullptr;
+    else
+        return psf + (size_t)z * 2 * (2 * R + 1);
+}
+
 __device__ __forceinline__ FastView load_fast_view(const float *rec)
 {
     FastView f;
@@ -322,7 +333,7 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
 
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
-        const float *kz = a.psf + (size_t)z * 2 * K;
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         float h[TS];
         {
             float yc[TS];
@@ -390,7 +401,7 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
 
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
-        const float *kz = a.psf + (size_t)z * 2 * K;
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         float yc[TS];
         if constexpr (HAS_MU) {
             resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
@@ -399,6 +410,9 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
                                                    return sv[j] * exp_neg(A[j]);
                                                },
                                                yc);
+        } else if constexpr (R == 0) {
+#pragma unroll
+            for (int i = 0; i < TS; i++) yc[i] = sv[i];
         } else {
 #pragma unroll
             for (int i = 0; i < TS; i++) {
@@ -447,7 +461,7 @@ __global__ void __launch_bounds__(512) k_grad_ref(EtSweepArgs a)
     }
     for (int z = a.nz - 1; z >= 0; z--) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
-        const float *kz = a.psf + (size_t)z * 2 * K;
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         // mu rows first: tail needs A(z') before this plane's mu joins the suffix
         float Az[NA];
         bilinear_map<NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, [&](int j, float m) {
@@ -496,7 +510,7 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
     }
     for (int z = a.nz - 1; z >= 0; z--) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
-        const float *kz = a.psf + (size_t)z * 2 * K;
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         float h[TS];
         {
             float yc[TS];

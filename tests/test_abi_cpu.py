@@ -1,0 +1,129 @@
+"""The C-ABI library without a GPU: it loads, exports every symbol that
+include/et_array_interface.h declares, validates arguments with the
+reference's status codes before touching a device, and reports a missing
+device with a status code instead of terminating the process."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from niftyrec_amd import _lib
+from oracle import refabi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "et_array_interface.h")
+
+
+@pytest.fixture(scope="module")
+def L():
+    return _lib.load()
+
+
+def _declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip and not n.isupper()})
+
+
+def test_exports_every_declared_symbol(L):
+    names = _declared()
+    assert len(names) >= 39
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_header():
+    """Every declared entry point has ctypes argument types in the binding."""
+    missing = [n for n in _declared() if n not in _lib.SIGNATURES]
+    assert not missing, missing
+
+
+def test_status_helpers_and_messages(L):
+    st = ctypes.c_int(-1)
+    assert L.status_parameter_error(ctypes.byref(st)) == 0 and st.value == 104
+    a, b = ctypes.c_int(7), ctypes.c_int(0)
+    assert L.echo(ctypes.byref(a), ctypes.byref(b)) == 0 and b.value == 7
+    assert b"inconsistent parameters" in L.niftyrec_error_msg(2)
+    assert b"no GPUs" in L.niftyrec_error_msg(8)
+    assert L.et_library_version().startswith(b"niftyrec-mi355x")
+
+
+def test_block_size_reports_any_size(L):
+    assert L.et_array_is_block_multiple(37) == 1
+    assert L.et_array_get_block_size() == 1
+
+
+def test_make_cameras_inclusive_endpoints(L):
+    """Host-side helper: theta_c = first + (last-first) c / (V-1), degrees -> rad
+    (_et_array_interface.cpp:923-937), placed on the rotation axis row."""
+    out = np.zeros((3, 5), np.float32)
+    assert L.et_array_make_cameras(out.ctypes.data_as(_lib.fp), 0.0, 180.0, 5, 1) == 0
+    assert np.allclose(out[1], np.deg2rad([0, 45, 90, 135, 180]), atol=1e-6)
+    assert not out[0].any() and not out[2].any()
+    if os.path.exists(refabi.REF_PATH):
+        R = refabi.load_ref()
+        ref = np.zeros((3, 5), np.float32)
+        R.et_array_make_cameras.argtypes = [_lib.fp, ctypes.c_float, ctypes.c_float, ctypes.c_uint, ctypes.c_uint]
+        R.et_array_make_cameras(ref.ctypes.data_as(_lib.fp), 0.0, 180.0, 5, 1)
+        assert np.array_equal(out, ref)
+
+
+def test_psf_stubs_match_reference(L):
+    sx, sy = ctypes.c_uint(0), ctypes.c_uint(0)
+    assert L.et_array_calculate_size_psf(ctypes.byref(sx), ctypes.byref(sy), 1, 1, 1, 1, 1, 1) == 0
+    assert (sx.value, sy.value) == (5, 5)
+    buf = np.zeros(5 * 5 * 3, np.float32)
+    assert L.et_array_make_psf(buf.ctypes.data_as(_lib.fp), 5, 5, 1, 1, 1, 1, 1, 1, 3) == 0
+    assert (buf == 1).all()
+    assert L.et_array_make_psf(buf.ctypes.data_as(_lib.fp), 3, 3, 1, 1, 1, 1, 1, 1, 3) == 1
+
+
+def _lib_abi():
+    return refabi.load(_lib.LIB_PATH)
+
+
+def test_parameter_errors_before_device():
+    """Validation of _et_array_interface.cpp:357-393 and :565-600 returns 2."""
+    A = _lib_abi()
+    act = np.ones((8, 6, 8), np.float32)
+    cams = np.zeros(4, np.float32)
+    # neither activity nor attenuation
+    sino = np.zeros((4, 6, 8), np.float32)
+    zero = np.zeros(3, np.int32)
+    st = A.SPECT_project_parallelholes(None, zero.ctypes.data_as(_lib.ip), sino.ctypes.data_as(_lib.fp),
+                                       np.array([8, 6, 4], np.int32).ctypes.data_as(_lib.ip),
+                                       cams.ctypes.data_as(_lib.fp), np.array([4, 1], np.int32).ctypes.data_as(_lib.ip),
+                                       None, zero.ctypes.data_as(_lib.ip), None, zero.ctypes.data_as(_lib.ip),
+                                       ctypes.byref(ctypes.c_float(0)), ctypes.byref(ctypes.c_float(0)),
+                                       ctypes.byref(ctypes.c_int(1)), ctypes.byref(ctypes.c_int(1)))
+    assert st == 2
+    # camera columns not in {1, 3, 6}
+    c2 = np.zeros((2, 4), np.float32)
+    assert refabi.project(A, act, None, c2)[0] == 2
+    # even PSF size
+    assert refabi.project(A, act, None, cams, np.ones((8, 4, 4), np.float32))[0] == 2
+    # PSF plane count != N
+    assert refabi.project(A, act, None, cams, np.ones((5, 3, 3), np.float32))[0] == 2
+    # backprojection: sinogram views != cameras
+    assert refabi.backproject(A, np.zeros((3, 6, 8), np.float32), None, cams)[0] == 2
+
+
+def test_no_device_is_a_status_not_an_exit():
+    """Without a HIP device every compute call returns 8 (no GPU installed)."""
+    L = _lib.load()
+    cnt = np.zeros(1, np.int32)
+    info = np.zeros(80, np.int32)
+    if L.et_array_list_gpus(cnt.ctypes.data_as(_lib.ip), info.ctypes.data_as(_lib.ip)) == 0:
+        pytest.skip("a GPU is present")
+    A = _lib_abi()
+    act = np.ones((8, 6, 8), np.float32)
+    assert refabi.project(A, act, None, np.zeros(4, np.float32))[0] == 8
+    assert refabi.backproject(A, np.zeros((4, 6, 8), np.float32), None, np.zeros(4, np.float32))[0] == 8
+    assert L.et_array_set_gpu(0) == 8
+    handle = ctypes.c_void_p()
+    assert L.et_plan_create(ctypes.byref(handle), 8, 6, 8, np.zeros(4, np.float32).ctypes.data_as(_lib.fp), 4, 1,
+                            None, 0, 0.0, 0.0) == 8

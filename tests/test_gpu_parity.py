@@ -1,0 +1,183 @@
+"""Parity of the HIP library (called through the NiftyRec C ABI) with the
+oracle, whose single-precision variant is bit-identical to the reference CPU
+build (tests/test_oracle_reference.py).
+
+Gate (single precision, from the project's accuracy target): relative L2
+error <= 1e-5 and max relative error <= 1e-4 (elements below 1e-4 of the peak
+measured against 1e-4 of the peak, see conftest.max_rel) against the reference
+restatement.  Where a double-precision oracle result is given, the HIP result
+must also be at least as close to it as the reference itself is (with a 1e-7
+allowance); both errors are printed.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import max_rel, rel_l2
+from niftyrec_amd import phantoms
+from oracle import orc, refabi
+
+pytestmark = pytest.mark.gpu
+
+RTOL_L2 = 1e-5
+RTOL_MAX = 1e-4
+
+
+def _case(shape=(32, 24, 32), V=16, K=5, seed=0, mu_scale=0.1):
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = shape
+    act, mu = phantoms.phantom(shape, 3, seed, lesion_axes=(1.5, 4.0))
+    act = act + 0.1 * rng.random(shape, dtype=np.float32)
+    mu = (mu / 0.15 * mu_scale).astype(np.float32)
+    cams = rng.uniform(0, 2 * np.pi, V).astype(np.float32)
+    psf = phantoms.gaussian_psf(nx, K, 0.5, 1.6) if K else None
+    return act, mu, cams, psf
+
+
+def _report(name, got, ref32, ref64=None):
+    e32, m32 = rel_l2(got, ref32), max_rel(got, ref32)
+    msg = "%s: relL2 %.2e max_rel %.2e vs reference" % (name, e32, m32)
+    if ref64 is not None:
+        msg += "; relL2 %.2e vs FP64 oracle (reference itself: %.2e)" % (rel_l2(got, ref64), rel_l2(ref32, ref64))
+    print(msg)
+    assert e32 <= RTOL_L2, msg
+    assert m32 <= RTOL_MAX, msg
+    if ref64 is not None:
+        assert rel_l2(got, ref64) <= 2.0 * rel_l2(ref32, ref64) + 1e-7, msg
+
+
+@pytest.mark.parametrize("use_mu", [True, False])
+@pytest.mark.parametrize("K", [0, 3, 5, 11])
+def test_project_parity(lib, use_mu, K):
+    act, mu, cams, psf = _case(K=K)
+    m = mu if use_mu else None
+    st, got = refabi.project(lib, act, m, cams, psf)
+    assert st == 0
+    _report("project K=%d mu=%d" % (K, use_mu), got, orc.project(act, m, cams, psf),
+            orc.project(act, m, cams, psf, dtype=np.float64))
+
+
+@pytest.mark.parametrize("use_mu", [True, False])
+@pytest.mark.parametrize("K", [0, 5, 11])
+def test_backproject_parity(lib, use_mu, K):
+    act, mu, cams, psf = _case(K=K, seed=1)
+    m = mu if use_mu else None
+    sino = orc.project(act, m, cams, psf)
+    st, got = refabi.backproject(lib, sino, m, cams, psf)
+    assert st == 0
+    _report("backproject K=%d mu=%d" % (K, use_mu), got, orc.backproject(sino, m, cams, psf),
+            orc.backproject(sino, m, cams, psf, dtype=np.float64))
+
+
+def test_project_odd_shapes(lib):
+    """Sizes that are not multiples of any block or slab size."""
+    for shape in [(21, 13, 21), (40, 37, 40), (7, 5, 7)]:
+        act, mu, cams, psf = _case(shape=shape, V=5, K=3, seed=shape[1])
+        st, got = refabi.project(lib, act, mu, cams, psf)
+        assert st == 0
+        _report("project %s" % (shape,), got, orc.project(act, mu, cams, psf))
+        st, bp = refabi.backproject(lib, got, mu, cams, psf)
+        assert st == 0
+        _report("backproject %s" % (shape,), bp, orc.backproject(got, mu, cams, psf))
+
+
+def test_background_nonzero(lib):
+    """bg != 0 reproduces the reference resampler's background rule."""
+    act, mu, cams, psf = _case(seed=2)
+    st, got = refabi.project(lib, act, mu, cams, psf, bg=0.5, bga=0.02)
+    assert st == 0
+    _report("project bg", got, orc.project(act, mu, cams, psf, 0.5, 0.02))
+    st, bp = refabi.backproject(lib, got, mu, cams, psf, bg=0.5, bga=0.02)
+    assert st == 0
+    _report("backproject bg", bp, orc.backproject(got, mu, cams, psf, 0.5, 0.02))
+
+
+def test_three_axis_cameras(lib):
+    """General rotations (3-column cameras) take the materialising path."""
+    act, mu, cams, psf = _case(seed=3, V=6)
+    rng = np.random.default_rng(3)
+    c3 = rng.uniform(-0.6, 0.6, (3, 6)).astype(np.float32)
+    st, got = refabi.project(lib, act, mu, c3, psf)
+    assert st == 0
+    _report("project 3-axis", got, orc.project(act, mu, c3, psf))
+    st, bp = refabi.backproject(lib, got, mu, c3, psf)
+    assert st == 0
+    _report("backproject 3-axis", bp, orc.backproject(got, mu, c3, psf))
+
+
+def test_wide_psf_general_path(lib):
+    """PSF radius above the sweep kernels' limit (K = 25)."""
+    act, mu, cams, psf = _case(shape=(32, 16, 32), V=4, K=25, seed=4)
+    st, got = refabi.project(lib, act, mu, cams, psf)
+    assert st == 0
+    _report("project K=25", got, orc.project(act, mu, cams, psf))
+    st, bp = refabi.backproject(lib, got, mu, cams, psf)
+    assert st == 0
+    _report("backproject K=25", bp, orc.backproject(got, mu, cams, psf))
+
+
+def test_attenuation_only_projection(lib):
+    """Activity absent: p = bg exp(-sum mu) (the reference CUDA path's result)."""
+    act, mu, cams, psf = _case(seed=5)
+    st, got = refabi.project(lib, None, mu, cams, None, bg=2.0)
+    assert st == 0
+    _report("project mu-only", got, orc.project(None, mu, cams, None, 2.0))
+
+
+@pytest.mark.parametrize("K", [0, 5])
+def test_gradient_reference(lib, K):
+    act, mu, cams, psf = _case(K=K, seed=6)
+    s = np.random.default_rng(6).random((len(cams),) + act.shape[1:], dtype=np.float32)
+    st, got = refabi.gradient_attenuation(lib, s, act, mu, cams, psf)
+    assert st == 0
+    _report("gradient ref K=%d" % K, got, orc.gradient_ref(s, act, mu, cams, psf),
+            orc.gradient_ref(s, act, mu, cams, psf, dtype=np.float64))
+
+
+def test_mlem_step(lib):
+    act, mu, cams, psf = _case(seed=7)
+    y = orc.project(act, mu, cams, psf)
+    x0 = np.ones_like(act)
+    st, got = refabi.mlem_step(lib, x0, y, mu, cams, psf)
+    assert st == 0
+    _report("mlem step", got, orc.mlem_step(x0, y, mu, cams, psf),
+            orc.mlem_step(x0, y, mu, cams, psf, dtype=np.float64))
+
+
+def test_osem_step_random_subset(lib):
+    """et_array_osem_step draws its subset from glibc rand(): seed it, replay the
+    draw in the oracle and compare with the MLEM update on that subset."""
+    act, mu, cams, psf = _case(seed=8)
+    y = orc.project(act, mu, cams, psf)
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1234)
+    st, got = refabi.osem_step(lib, np.ones_like(act), y, mu, cams, 4, psf)
+    assert st == 0
+    libc.srand(1234)
+    idx = orc.osem_subset(len(cams), len(cams) // 4)
+    ref = orc.mlem_step(np.ones_like(act), y[idx], mu, cams[idx], psf)
+    _report("osem step", got, ref)
+
+
+def test_rotate(lib):
+    rng = np.random.default_rng(9)
+    img = rng.random((20, 14, 18), dtype=np.float32)
+    ctr = np.array([8.5, 6.5, 9.5], np.float32)
+    for order in range(6):
+        ang = rng.uniform(-1, 1, 3).astype(np.float32)
+        st, got = refabi.rotate(lib, img, ang, ctr, 0.25, order)
+        assert st == 0
+        _report("rotate order %d" % order, got, orc.resample(img, orc.rotation_matrix(ang, ctr, order), 0.25))
+
+
+def test_c1_full_size(lib):
+    """Full config-1 shapes: 64^3, 64 views, PSF 7, attenuation."""
+    w = phantoms.workload("c1")
+    act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    st, got = refabi.project(lib, act, mu, cams, psf)
+    assert st == 0
+    _report("c1 project", got, orc.project(act, mu, cams, psf), orc.project(act, mu, cams, psf, dtype=np.float64))
+    st, bp = refabi.backproject(lib, got, mu, cams, psf)
+    assert st == 0
+    _report("c1 backproject", bp, orc.backproject(got, mu, cams, psf))

@@ -1,0 +1,238 @@
+"""Pins of the oracle that need no reference run: closed forms, invariants,
+library routines it reduces to, dense brute force and finite differences.
+Each test is chosen so that a plausible slip (dropped term, wrong sign or
+index, transposed operand, exclusive instead of inclusive sum) fails it."""
+import numpy as np
+import pytest
+from scipy.signal import convolve2d
+
+from oracle import orc
+
+
+def test_rotation_identity_and_orthogonality():
+    M = orc.rotation_matrix([0, 0, 0], [3.5, 2.0, 3.5], 0, dtype=np.float64)
+    assert np.array_equal(M, np.eye(4))
+    rng = np.random.default_rng(0)
+    for order in range(6):
+        a = rng.uniform(-np.pi, np.pi, 3)
+        M = orc.rotation_matrix(a, [4.0, 5.0, 6.0], order, dtype=np.float64)
+        R = M[:3, :3]
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-14)
+        assert np.isclose(np.linalg.det(R), 1.0)
+        # the centre is a fixed point
+        assert np.allclose(M @ np.array([4.0, 5.0, 6.0, 1.0]), [4.0, 5.0, 6.0, 1.0])
+
+
+def test_rotation_xyz_composition_and_inverse():
+    """XYZ order is Rz Ry Rx; negated angles in ZYX order give the inverse."""
+    ax, ay, az = 0.3, -0.7, 1.1
+    c, s = np.cos, np.sin
+    Rx = np.array([[1, 0, 0], [0, c(ax), -s(ax)], [0, s(ax), c(ax)]])
+    Ry = np.array([[c(ay), 0, s(ay)], [0, 1, 0], [-s(ay), 0, c(ay)]])
+    Rz = np.array([[c(az), -s(az), 0], [s(az), c(az), 0], [0, 0, 1]])
+    M = orc.rotation_matrix([ax, ay, az], [0, 0, 0], 0, dtype=np.float64)
+    assert np.allclose(M[:3, :3], Rz @ Ry @ Rx, atol=1e-15)
+    Mi = orc.rotation_matrix([-ax, -ay, -az], [1, 2, 3], 5, dtype=np.float64)
+    Mf = orc.rotation_matrix([ax, ay, az], [1, 2, 3], 0, dtype=np.float64)
+    assert np.allclose(Mi @ Mf, np.eye(4), atol=1e-14)
+
+
+def test_resample_identity_and_quarter_turn():
+    rng = np.random.default_rng(1)
+    v = rng.random((9, 5, 9))
+    assert np.array_equal(orc.resample(v, np.eye(4), 0.0, dtype=np.float64), v)
+    # quarter turn about y, centre 4: sample(x', y, z') = v(z', y, 8 - x') ... derived from
+    # p = R (x'-c) + c with R = Ry(pi/2): px = (z'-4) + 4, pz = -(x'-4) + 4
+    M = orc.rotation_matrix([0, np.pi / 2, 0], [4, 2, 4], 0, dtype=np.float64)
+    out = orc.resample(v, M, 0.0, dtype=np.float64)
+    z, y, x = np.meshgrid(np.arange(9), np.arange(5), np.arange(9), indexing="ij")
+    expect = v[8 - x, y, z]
+    assert np.allclose(out, expect, atol=1e-12)
+
+
+def test_resample_half_voxel_shift_and_border():
+    rng = np.random.default_rng(2)
+    v = rng.random((4, 3, 6))
+    M = np.eye(4)
+    M[0, 3] = 0.5  # sample at x + 0.5
+    out = orc.resample(v, M, 0.0, dtype=np.float64)
+    assert np.allclose(out[..., :-1], 0.5 * (v[..., :-1] + v[..., 1:]), atol=1e-14)
+    # last column: the x+1 neighbour is outside -> zero weight contribution
+    assert np.allclose(out[..., -1], 0.5 * v[..., -1], atol=1e-14)
+    # background rule at bg != 0 (reg-lib/_reg_resampling.cpp:366-396): every
+    # in-range z plane adds bg*xBasis[1] for the missing x neighbour in row y and
+    # again in row y+1 (even though the y weight of that row is 0); the missing
+    # z plane z+1 of the last slice adds bg*xBasis[1] (the x basis, indexed by c)
+    outb = orc.resample(v, M, 2.0, dtype=np.float64)
+    nz = v.shape[0]
+    extra = np.where(np.arange(nz) < nz - 1, 4 * (2.0 * 0.5), 3 * (2.0 * 0.5))[:, None]
+    assert np.allclose(outb[:, :, -1], 0.5 * v[:, :, -1] + extra, atol=1e-12)
+
+
+def test_psf_separable_equals_2d_convolution():
+    """Separated PSF (central row, central column / centre) applied row then
+    column equals scipy's 2-D convolution with the full separable kernel, for a
+    kernel that is not symmetric (pins the convolution orientation)."""
+    rng = np.random.default_rng(3)
+    nz, ny, nx, h = 3, 11, 13, 5
+    vol = rng.random((nz, ny, nx))
+    psf = np.empty((nz, h, h))
+    for d in range(nz):
+        a = rng.random(h) + 0.1   # along y (rows)
+        b = rng.random(h) + 0.1   # along x (columns)
+        psf[d] = np.outer(a, b)
+    out = orc.convolve_planes(vol, psf, dtype=np.float64)
+    for d in range(nz):
+        assert np.allclose(out[d], convolve2d(vol[d], psf[d], mode="same", boundary="fill"), atol=1e-12)
+    # the transpose flag gives the adjoint: <C x, y> = <x, C^T y>
+    yv = rng.random(vol.shape)
+    lhs = np.vdot(out, yv)
+    rhs = np.vdot(vol, orc.convolve_planes(yv, psf, transpose=True, dtype=np.float64))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_delta_psf_is_identity():
+    rng = np.random.default_rng(4)
+    vol = rng.random((4, 6, 7))
+    psf = np.zeros((4, 3, 3))
+    psf[:, 1, 1] = 1.0
+    assert np.allclose(orc.convolve_planes(vol, psf, dtype=np.float64), vol, atol=0)
+
+
+def test_line_integral_closed_form_inclusive():
+    """Uniform activity a and attenuation mu over L voxels at theta = 0:
+    p = a sum_{k=1..L} e^{-k mu} + bg e^{-L mu} (the voxel's own mu attenuates it)."""
+    L, a, mu, bg = 12, 2.5, 0.07, 0.3
+    act = np.full((L, 3, 4), a)
+    att = np.full((L, 3, 4), mu)
+    p = orc.project(act, att, np.zeros(1), None, bg, 0.0, 0, dtype=np.float64)
+    # bg also pads the rotation of act; at theta = 0 no neighbour is missing
+    # except x+1 at the last column and y+1 at the last row, whose basis is 0
+    expect = a * np.exp(-mu) * (1 - np.exp(-L * mu)) / (1 - np.exp(-mu)) + bg * np.exp(-L * mu)
+    assert np.allclose(p, expect, rtol=1e-13)
+    # no attenuation: plain column sum plus bg
+    p0 = orc.project(act, None, np.zeros(1), None, bg, 0.0, 0, dtype=np.float64)
+    assert np.allclose(p0, L * a + bg, rtol=1e-14)
+
+
+def test_line_backprojection_closed_form():
+    """One view at theta = 0, no PSF: b(z) = s e^{-(z+1) mu}."""
+    L, mu = 10, 0.05
+    s = np.random.default_rng(5).random((1, 3, 4))
+    att = np.full((L, 3, 4), mu)
+    b = orc.backproject(s, att, np.zeros(1), None, 0.0, 0.0, 0, nz=L, dtype=np.float64)
+    z = np.arange(L)[:, None, None]
+    assert np.allclose(b, s[0][None] * np.exp(-(z + 1) * mu), rtol=1e-13)
+
+
+def test_reference_gradient_closed_form():
+    """K9 semantics on a uniform column at theta = 0, no PSF:
+    g(z) = s a sum_{k>=z} e^{-(k+1) mu}."""
+    L, a, mu = 9, 1.7, 0.11
+    act = np.full((L, 2, 3), a)
+    att = np.full((L, 2, 3), mu)
+    s = np.full((1, 2, 3), 0.8)
+    g = orc.gradient_ref(s, act, att, np.zeros(1), None, 0.0, 0.0, 0, dtype=np.float64)
+    for z in range(L):
+        expect = 0.8 * a * sum(np.exp(-(k + 1) * mu) for k in range(z, L))
+        assert np.allclose(g[z], expect, rtol=1e-13)
+
+
+def _small_problem(seed=6, shape=(7, 5, 7), V=5, K=3):
+    rng = np.random.default_rng(seed)
+    act = rng.random(shape)
+    mu = 0.05 * rng.random(shape)
+    cams = rng.uniform(0, 2 * np.pi, V)
+    from niftyrec_amd.phantoms import gaussian_psf
+    psf = gaussian_psf(shape[2], K, 0.6, 1.2).astype(np.float64)
+    # make the PSF asymmetric so correlation and convolution differ
+    psf[:, :, 0] *= 1.3
+    return act, mu, cams, psf
+
+
+def test_adjoint_dense_brute_force():
+    """The adjoint backprojection equals the transpose of the dense matrix of
+    the projector (columns built from unit activities)."""
+    act, mu, cams, psf = _small_problem()
+    shape = act.shape
+    n = act.size
+    cols = []
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        cols.append(orc.project(e.reshape(shape), mu, cams, psf, 0.0, 0.0, 0, dtype=np.float64).ravel())
+    A = np.stack(cols, axis=1)
+    y = np.random.default_rng(7).random(A.shape[0])
+    At_y = orc.backproject(y.reshape((len(cams),) + shape[1:]), mu, cams, psf, 0.0, 0.0, 0, mode=1,
+                           nz=shape[0], dtype=np.float64)
+    assert np.allclose(At_y.ravel(), A.T @ y, rtol=1e-12, atol=1e-13)
+    # and the reference backprojector is not that transpose at general angles
+    ref_bp = orc.backproject(y.reshape((len(cams),) + shape[1:]), mu, cams, psf, 0.0, 0.0, 0, mode=0,
+                             nz=shape[0], dtype=np.float64)
+    assert not np.allclose(ref_bp.ravel(), A.T @ y, rtol=1e-3)
+
+
+def test_dot_product_identity_fp64():
+    act, mu, cams, psf = _small_problem(seed=8, shape=(12, 9, 12), V=9, K=5)
+    y = np.random.default_rng(9).random((len(cams),) + act.shape[1:])
+    Ax = orc.project(act, mu, cams, psf, 0.0, 0.0, 0, dtype=np.float64)
+    Aty = orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1, nz=act.shape[0], dtype=np.float64)
+    lhs, rhs = np.vdot(Ax, y), np.vdot(act, Aty)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+
+def test_gradient_mu_finite_differences():
+    """d/dmu of <s, project(act, mu)> against FP64 central differences (step 1e-3)."""
+    act, mu, cams, psf = _small_problem(seed=10, shape=(8, 6, 8), V=6, K=3)
+    s = np.random.default_rng(11).random((len(cams),) + act.shape[1:])
+    bg = 0.4
+    g = orc.gradient_mu(s, act, mu, cams, psf, bg, 0.0, dtype=np.float64)
+    rng = np.random.default_rng(12)
+    idx = rng.choice(mu.size, 24, replace=False)
+    h = 1e-3
+    for i in idx:
+        mp, mm = mu.copy().ravel(), mu.copy().ravel()
+        mp[i] += h
+        mm[i] -= h
+        fp = np.vdot(s, orc.project(act, mp.reshape(mu.shape), cams, psf, bg, 0.0, 0, dtype=np.float64))
+        fm = np.vdot(s, orc.project(act, mm.reshape(mu.shape), cams, psf, bg, 0.0, 0, dtype=np.float64))
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - g.ravel()[i]) <= 1e-6 * max(abs(fd), 1e-3), (i, fd, g.ravel()[i])
+
+
+def test_mlem_fixed_point_and_update_rule():
+    """If y = A x*, x* is stationary; the update is x * A^T(y/Ax) / A^T 1."""
+    act, mu, cams, psf = _small_problem(seed=13)
+    act = act + 0.5
+    y = orc.project(act, mu, cams, psf, 0.0, 0.0, 1, dtype=np.float64)
+    x1 = orc.mlem_step(act, y, mu, cams, psf, eps=1e-9, dtype=np.float64)
+    assert np.allclose(x1, act, rtol=1e-12)
+    x0 = np.ones_like(act)
+    p = orc.project(x0, mu, cams, psf, 0.0, 0.0, 1, dtype=np.float64)
+    b = orc.backproject(y / p, mu, cams, psf, 0.0, 0.0, 1, nz=act.shape[0], dtype=np.float64)
+    sens = orc.backproject(np.ones_like(y), mu, cams, psf, 0.0, 0.0, 1, nz=act.shape[0], dtype=np.float64)
+    expect = x0 * b / np.maximum(sens, 1e-9)
+    assert np.allclose(orc.mlem_step(x0, y, mu, cams, psf, eps=1e-9, dtype=np.float64), expect, rtol=1e-12)
+
+
+def test_single_precision_tracks_double():
+    act, mu, cams, psf = _small_problem(seed=14, shape=(16, 12, 16), V=8, K=5)
+    p32 = orc.project(act, mu, cams, psf, dtype=np.float32)
+    p64 = orc.project(act, mu, cams, psf, dtype=np.float64)
+    assert np.linalg.norm(p32 - p64) / np.linalg.norm(p64) < 1e-6
+
+
+def test_osem_subset_draw_matches_formula():
+    """Two rand() calls per camera, R = base + fine / 2^31 in float, index = V*R."""
+    import ctypes
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(5)
+    idx = orc.osem_subset(40, 10)
+    libc.srand(5)
+    expect = []
+    for _ in range(10):
+        base = np.float32(np.float32(libc.rand()) / np.float32(2.0 ** 31))
+        fine = np.float32(np.float32(libc.rand()) / np.float32(2.0 ** 31))
+        R = np.float32(base + np.float32(fine / np.float32(2.0 ** 31)))
+        expect.append(min(int(np.float32(40) * R), 39))
+    assert list(idx) == expect
