@@ -1,0 +1,139 @@
+"""Benchmark: OSEM iterations/s for SPECT at 128^3 x 120 views, depth-dependent
+PSF 11x11 and attenuation, 8 ordered subsets (the config-2 workload), on one
+MI355X with every input resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+A step is one OSEM iteration: 8 subset updates, each = projection of the
+subset's 15 views -> ratio -> reference backprojection -> multiplicative
+update (sensitivities cached).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "OSEM iters/s at 128³×120 views PSF+atten; GVoxel-views/s; % HBM peak"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="c2")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, subsets):
+    """Reference CPU path (oracle/_ref, single thread as built by the reference)
+    on one subset update of the same workload; falls back to the oracle port."""
+    from oracle import refabi, orc
+    a, m, p, c = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    views = c[0::subsets]  # the first ordered subset: 15 views
+    y = np.ones((len(views),) + a.shape[1:], np.float32)
+    if os.path.exists(refabi.REF_PATH):
+        R = refabi.load_ref()
+        t = time.perf_counter()
+        st, _ = refabi.mlem_step(R, np.ones_like(a), y, m, views, p)
+        dt = time.perf_counter() - t
+        kind, cores = "reference", 1
+        assert st == 0
+    else:
+        t = time.perf_counter()
+        orc.mlem_step(np.ones_like(a), y, m, views, p)
+        dt = time.perf_counter() - t
+        kind, cores = "port", os.cpu_count()
+    it_time = dt * subsets  # one iteration = `subsets` subset updates of this size
+    return {"value": 1.0 / it_time, "unit": "OSEM it/s", "cores": cores, "kind": kind,
+            "sample": "one subset update (et_array_mlem_step on %d of %d views, 128^3, PSF 11, attenuation; "
+                      "the reference recomputes the sensitivity) took %.2f s; iteration = %d such updates"
+                      % (len(views), len(c), dt, subsets)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (use --gpus 1)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from niftyrec_amd import api, phantoms
+
+    w = phantoms.workload(args.workload)
+    S = w["subsets"] if args.workload == "c2" else 1
+    act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    nz, ny, nx = act.shape
+    V = len(cams)
+    dev = torch.device("cuda:0")
+    A = torch.from_numpy(act).to(dev)
+    M = torch.from_numpy(mu).to(dev)
+    plan = api.Plan(act.shape, cams, psf)
+    noiseless = plan.project(A, M).cpu().numpy()
+    Y = torch.from_numpy(phantoms.poisson_data(noiseless, w["counts"], seed=1)).to(dev)
+    X = torch.ones_like(A)
+
+    # warmup (first call builds the cached sensitivities)
+    plan.osem(X, Y, M, subsets=S, iterations=max(args.warmup, 1))
+    torch.cuda.synchronize()
+
+    X.fill_(1.0)
+    plan.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.osem(X, Y, M, subsets=S, iterations=1)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = plan.profile_read()
+    plan.profile(False)
+    assert torch.isfinite(X).all()
+
+    ms = elapsed / args.steps * 1e3
+    its = args.steps / elapsed
+    n = nx * ny * nz
+    P = nx * ny
+    bytes_iter = V * (16.0 * n + 8.0 * P) + S * 16.0 * n  # BASELINE.md section 2, sensitivity cached
+    # dominant kernel: the category with the largest total time
+    cat = max((k for k in prof if prof[k][0] > 0), key=lambda k: prof[k][1])
+    launches, tot_ms, tot_bytes = prof[cat]
+    achieved = tot_bytes / (tot_ms * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": its,
+        "unit": "OSEM it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, Poisson counts %.0e)" % w["counts"],
+        "config": {"workload": "%s: %dx%dx%d activity+attenuation, %d views over 360 deg, depth-dependent "
+                               "Gaussian PSF %dx%d, %d ordered subsets x 1 iteration per step"
+                               % (args.workload, nx, ny, nz, V, psf.shape[-1], psf.shape[-1], S),
+                   "volume": [nx, ny, nz], "views": V, "psf": int(psf.shape[-1]), "subsets": S},
+        "gvoxel_views_per_s": V * n / (ms * 1e-3) / 1e9,
+        "hbm_frac_of_peak": bytes_iter / (ms * 1e-3) / HBM_PEAK,
+        "roofline": {"bound": "hbm", "kernel": cat, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "avg_launch_ms": tot_ms / launches, "algorithmic_bytes_per_launch": tot_bytes / launches},
+        "kernel_ms_per_step": {k: v[1] / args.steps for k, v in prof.items() if v[0]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, S)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -222,9 +222,9 @@ struct et_plan {
     int gy = ET_GY, yp = 0, xp = 0, zp = 0, yu = 0, slabs = 0;
     size_t yf_elems = 0, ustride = 0, P = 0, nvox = 0;
     // workspaces
-    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, csum, astart, atot, tmp, tmp2, em_sino, em_vol;
+    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol;
     // OSEM caches
-    DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart;
+    DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart, osem_order;
     int osem_S = 0, osem_D = 0;
     const float *osem_mu_key = nullptr;
     float osem_eps = 0.f;
@@ -396,17 +396,18 @@ EtSweepArgs base_args(et_plan *p)
 }
 
 // Depth chunks so that a launch over `views` views fills the device
-// (about 4 waves per SIMD of 256 CUs), each chunk at least 8 planes deep.
+// (about 2 waves per SIMD of 256 CUs), each chunk at least 16 planes deep:
+// more, shorter chunks cost more in per-block set-up than they gain.
 int choose_D(et_plan *p, int views)
 {
     if (p->tune_D > 0) return std::min(p->tune_D, p->nz);
     const int waves_per_block = nthreads_for(p->nx) / 64;
     const int ts = et_sweep_slab_rows(ET_MODE_PROJECT, p->R);
     const long slabs = (p->ny + ts - 1) / ts;
-    const long target = 4L * 1024;
+    const long target = 2L * 1024;
     const long waves = slabs * views * waves_per_block;
     int D = 1;
-    while (waves * D < target && p->nz / (D * 2) >= 8) D *= 2;
+    while (waves * D < target && p->nz / (D * 2) >= 16) D *= 2;
     return D;
 }
 
@@ -771,8 +772,13 @@ void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const 
 {
     p->u.ensure((size_t)nv * p->ustride, true, p->st);
     float *parts = p->part.ensure((size_t)D * nv * p->P, false);
+    float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
     project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, D, astart, parts);
-    bpu_fast(p, recs, nv, mu_yf, ET_SRC_RATIO, nullptr, meas, parts, D, eps, D, astart, false, p->u.p);
+    {
+        ProfScope ps(p, ET_PROF_OTHER, (D + 2.0) * 4.0 * nv * p->P);
+        check_launch(et_launch_ratio_parts(meas, parts, D, (size_t)nv * p->P, eps, ratio, (et_stream_t)p->st));
+    }
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, D, astart, false, p->u.p);
     pull_fast(p, recs, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, x_yf, sens_yf);
 }
 
@@ -840,6 +846,10 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
         check_hip(hipMemcpyAsync(p->osem_recs.p, recs.data(), sizeof(float) * recs.size(), hipMemcpyHostToDevice,
                                  p->st),
                   niftyrec_error_transfergpu);
+        p->osem_order.ensure(order.size(), false);
+        check_hip(hipMemcpyAsync(p->osem_order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice,
+                                 p->st),
+                  niftyrec_error_transfergpu);
         p->sens.ensure((size_t)S * p->yf_elems, true, p->st);
         if (mu_yf && D > 1) p->osem_astart.ensure((size_t)p->V * D * p->P, false);
         for (int s = 0; s < S; s++) {
@@ -858,12 +868,13 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
         p->osem_mu_key = mu;
         p->osem_eps = eps;
     }
-    // measured data in subset order (device to device copies)
+    // measured data in subset order
     p->osem_meas.ensure((size_t)p->V * p->P, false);
-    for (int i = 0; i < p->V; i++)
-        check_hip(hipMemcpyAsync(p->osem_meas.p + (size_t)i * p->P, meas + (size_t)order[i] * p->P,
-                                 sizeof(float) * p->P, hipMemcpyDeviceToDevice, p->st),
-                  niftyrec_error_transfergpu);
+    {
+        ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->V * p->P);
+        check_launch(et_launch_gather_planes(p->osem_meas.p, meas, reinterpret_cast<const int *>(p->osem_order.p),
+                                             p->V, p->P, (et_stream_t)p->st));
+    }
     pack(p, x, p->x_yf);
     for (int it = 0; it < iters; it++)
         for (int s = 0; s < S; s++) {

@@ -125,6 +125,10 @@ int et_launch_yf_clamp_min(float *yf, int nx, int ny, int nz, int yp, int gy, fl
 int et_launch_osem_yf_update(float *x_yf, const float *b_yf, const float *s_yf, int nx, int ny,
                              int nz, int yp, int gy, et_stream_t stream);
 int et_launch_em_update_flat(float *x, const float *b, const float *s, size_t n, et_stream_t stream);
+int et_launch_gather_planes(float *dst, const float *src, const int *order, int V, size_t P,
+                            et_stream_t stream);
+int et_launch_ratio_parts(const float *meas, const float *part, int D, size_t VP, float eps, float *out,
+                          et_stream_t stream);
 int et_launch_ratio(const float *meas, const float *proj, float *out, size_t n, float eps,
                     et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).

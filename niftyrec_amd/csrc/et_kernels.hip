@@ -102,69 +102,77 @@ __device__ __forceinline__ Samp make_samp(float px, float pz, int nx, int nz, in
     return s;
 }
 
-// Streams the bilinear samples of NA consecutive rows (starting SA rows into a
-// LA-long aligned run at p00) through the axial (column) pass of the PSF:
-//   yc[i] = sum_{j=0}^{2R} row[i+j] * kc[2R-j],  i < TS
+// Bilinear samples of NA consecutive rows (starting SA rows into a LA-long
+// aligned run at p00 of every corner) into out[NA], mapped through F(j, value).
+// Corner-major: the whole run of one corner (one or two cache lines per lane)
+// is loaded back to back and folded into out[] before the next corner, so a
+// line is fetched from L2 once while it is hot in L1; the fence keeps the
+// scheduler from hoisting the next corner's loads (register footprint).
+template <int NA, int LA, int SA, class F>
+__device__ __forceinline__ void bilinear_rows(const float *__restrict__ p00, int dx, int dz, const Samp &s, F f,
+                                              float *out)
+{
+    const float *cp[4] = {p00, p00 + dx, p00 + dz, p00 + dz + dx};
+    const float w[4] = {s.w00, s.w01, s.w10, s.w11};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float4 *q = reinterpret_cast<const float4 *>(cp[c]);
+        float run[LA];
+#pragma unroll
+        for (int k = 0; k < LA / 4; k++) {
+            if (4 * k + 3 < SA || 4 * k >= SA + NA) continue;
+            const float4 t = q[k];
+            run[4 * k] = t.x;
+            run[4 * k + 1] = t.y;
+            run[4 * k + 2] = t.z;
+            run[4 * k + 3] = t.w;
+        }
+#pragma unroll
+        for (int j = 0; j < NA; j++) out[j] = (c == 0) ? run[j + SA] * w[0] : fmaf(run[j + SA], w[c], out[j]);
+        ET_CHUNK_FENCE();
+    }
+#pragma unroll
+    for (int j = 0; j < NA; j++) out[j] = f(j, out[j]);
+}
+
+// Axial (column) pass of the PSF in registers:
+//   yc[i] = sum_{j=0}^{2R} in[i+j] * kc[2R-j],  i < TS
 // (convolution out[y] = sum_k in[y+k] kc[R-k], _et_convolveSeparable2D.cpp:28-39).
-// Each loaded chunk of 4 rows is consumed at once, so raw rows never stay live.
-// F(j, value) -> value maps a resampled row before it enters the pass.
+template <int TS, int R>
+__device__ __forceinline__ void column_pass(const float *in, const float *__restrict__ kc, float *yc)
+{
+#pragma unroll
+    for (int i = 0; i < TS; i++) {
+        if constexpr (R == 0) {
+            yc[i] = in[i];
+        } else {
+            float t = 0.f;
+#pragma unroll
+            for (int j = 0; j <= 2 * R; j++) t = fmaf(in[i + j], kc[2 * R - j], t);
+            yc[i] = t;
+        }
+    }
+}
+
+// Resample NA rows and apply the column pass (see bilinear_rows / column_pass).
 template <int TS, int R, int NA, int LA, int SA, class F>
 __device__ __forceinline__ void resample_column(const float *__restrict__ p00, int dx, int dz, const Samp &s,
                                                 const float *__restrict__ kc, F f, float *yc)
 {
-    const float4 *q00 = reinterpret_cast<const float4 *>(p00);
-    const float4 *q01 = reinterpret_cast<const float4 *>(p00 + dx);
-    const float4 *q10 = reinterpret_cast<const float4 *>(p00 + dz);
-    const float4 *q11 = reinterpret_cast<const float4 *>(p00 + dz + dx);
-#pragma unroll
-    for (int i = 0; i < TS; i++) yc[i] = 0.f;
-#pragma unroll
-    for (int k = 0; k < LA / 4; k++) {
-        if (4 * k + 3 < SA || 4 * k >= SA + NA) continue;
-        const float4 a = q00[k], b = q01[k], c = q10[k], d = q11[k];
-        const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
-        const float cv[4] = {c.x, c.y, c.z, c.w}, dv[4] = {d.x, d.y, d.z, d.w};
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int j = 4 * k + e - SA;
-            if (j < 0 || j >= NA) continue;
-            const float v = f(j, av[e] * s.w00 + bv[e] * s.w01 + cv[e] * s.w10 + dv[e] * s.w11);
-            if constexpr (R == 0) {
-                yc[j] = v;
-            } else {
-#pragma unroll
-                for (int i = 0; i < TS; i++) {
-                    const int t = j - i;  // tap index into the reversed kernel
-                    if (t >= 0 && t <= 2 * R) yc[i] = fmaf(v, kc[2 * R - t], yc[i]);
-                }
-            }
-        }
-        ET_CHUNK_FENCE();
-    }
+    float rows[NA];
+    bilinear_rows<NA, LA, SA>(p00, dx, dz, s, f, rows);
+    column_pass<TS, R>(rows, kc, yc);
 }
 
 // out[j] = bilinear sample of rows SKIP + j of a LOADED-long aligned run, j < NROWS,
-// mapped through F(j, value).
+// mapped through F(j, value) (corner-major, see bilinear_rows).
 template <int NROWS, int LOADED, int SKIP, class F>
 __device__ __forceinline__ void bilinear_map(const float *__restrict__ p00, int dx, int dz, const Samp &s, F f)
 {
-    const float4 *q00 = reinterpret_cast<const float4 *>(p00);
-    const float4 *q01 = reinterpret_cast<const float4 *>(p00 + dx);
-    const float4 *q10 = reinterpret_cast<const float4 *>(p00 + dz);
-    const float4 *q11 = reinterpret_cast<const float4 *>(p00 + dz + dx);
+    float rows[NROWS];
+    bilinear_rows<NROWS, LOADED, SKIP>(p00, dx, dz, s, [](int, float v) { return v; }, rows);
 #pragma unroll
-    for (int k = 0; k < LOADED / 4; k++) {
-        if (4 * k + 3 < SKIP || 4 * k >= SKIP + NROWS) continue;
-        const float4 a = q00[k], b = q01[k], c = q10[k], d = q11[k];
-        const float av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
-        const float cv[4] = {c.x, c.y, c.z, c.w}, dv[4] = {d.x, d.y, d.z, d.w};
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int j = 4 * k + e - SKIP;
-            if (j >= 0 && j < NROWS) f(j, av[e] * s.w00 + bv[e] * s.w01 + cv[e] * s.w10 + dv[e] * s.w11);
-        }
-        ET_CHUNK_FENCE();
-    }
+    for (int j = 0; j < NROWS; j++) f(j, rows[j]);
 }
 
 // Row (x) pass through LDS for a block whose thread tx owns column x' = tx:
@@ -1001,6 +1009,29 @@ __global__ void __launch_bounds__(256) k_em_update_flat(float *__restrict__ x, c
     x[i] = x[i] * (bb / s[i]);
 }
 
+// dst[i] = src[order[i]] for planes of P floats (views reordered by subset).
+__global__ void __launch_bounds__(256) k_gather_planes(float *__restrict__ dst, const float *__restrict__ src,
+                                                      const int *__restrict__ order, int V, size_t P)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)V * P) return;
+    const int v = (int)(i / P);
+    dst[i] = src[(size_t)order[v] * P + (i % P)];
+}
+
+// OSEM ratio from partial projections, reference _et_array_interface.cpp:1098-1108:
+// p = max(sum_d part[d], 0); r = p < eps ? y / eps : y / p.
+__global__ void __launch_bounds__(256) k_ratio_parts(const float *__restrict__ meas, const float *__restrict__ part,
+                                                    int D, size_t VP, float eps, float *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= VP) return;
+    float p = 0.f;
+    for (int d = 0; d < D; d++) p += part[(size_t)d * VP + i];
+    if (p < 0.f) p = 0.f;
+    out[i] = (p < eps) ? meas[i] / eps : meas[i] / p;
+}
+
 inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -1215,5 +1246,20 @@ int et_launch_em_update_flat(float *x, const float *b, const float *s, size_t n,
 {
     hipLaunchKernelGGL(k_em_update_flat, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), x, b,
                        s, n);
+    return (int)hipGetLastError();
+}
+
+int et_launch_gather_planes(float *dst, const float *src, const int *order, int V, size_t P, et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_gather_planes, dim3(nblk((size_t)V * P, 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), dst, src, order, V, P);
+    return (int)hipGetLastError();
+}
+
+int et_launch_ratio_parts(const float *meas, const float *part, int D, size_t VP, float eps, float *out,
+                          et_stream_t stream)
+{
+    hipLaunchKernelGGL(k_ratio_parts, dim3(nblk(VP, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), meas,
+                       part, D, VP, eps, out);
     return (int)hipGetLastError();
 }
