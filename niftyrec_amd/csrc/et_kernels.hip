@@ -175,6 +175,95 @@ __device__ __forceinline__ void bilinear_map(const float *__restrict__ p00, int 
     for (int j = 0; j < NROWS; j++) f(j, rows[j]);
 }
 
+// Orders LDS accesses of the lanes of one wave (compiler and memory fence at
+// wavefront scope; the LDS executes one wave's instructions in order).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Sample table of one wave for coop_rows (structure of arrays: offset bits,
+// w00, w01, w10, w11; 64 entries each).
+constexpr int ET_COOP_TAB = 5 * 64;
+
+// Largest number of owners whose runs of LA rows fit a stage of `floats`
+// floats (row stride LA + 4), rounded down to whole rounds of 64/(LA/4) owners.
+__host__ __device__ constexpr int coop_owners(int LA, int floats)
+{
+    return (floats / (LA + 4)) >= 64 ? 64 : ((floats / (LA + 4)) / (64 / (LA / 4))) * (64 / (LA / 4));
+}
+
+// Wave-cooperative form of bilinear_rows; every lane of the wave must call it.
+// A lane-private gather touches a different cache line in every lane of every
+// load instruction, and the L1 processes those one lane at a time.  Here each
+// lane publishes its sample (corner offset, four weights) in `tab`; then, round
+// by round, the CH = LA/4 consecutive lanes serving one owner load the CH
+// consecutive 16-byte chunks of that owner's four corner runs (coalesced),
+// combine them with the owner's weights and store the owner's bilinear rows to
+// `stage` (OPP owners at a time, row stride LA + 4), from which the owners read
+// their runs back.  `stage` and `tab` belong to the calling wave alone.
+template <int NA, int LA, int SA, int OPP, class F>
+__device__ __forceinline__ void coop_rows(const float *__restrict__ base, int dx, int dz, const Samp &s,
+                                          float *tab, float *stage, F f, float *out)
+{
+    constexpr int CH = LA / 4;
+    constexpr int RPR = 64 / CH;
+    constexpr int RS = LA + 4;
+    constexpr int PH = (64 + OPP - 1) / OPP;
+    constexpr int RPP = (OPP + RPR - 1) / RPR;
+    static_assert(CH <= 64 && OPP >= 1 && OPP <= 64, "bad cooperative shape");
+    const int lane = threadIdx.x & 63;
+    tab[lane] = __int_as_float(s.off);
+    tab[64 + lane] = s.w00;
+    tab[128 + lane] = s.w01;
+    tab[192 + lane] = s.w10;
+    tab[256 + lane] = s.w11;
+    wave_sync();
+    const int k = lane % CH, tl = lane / CH;
+    const bool serve = (tl < RPR) && (4 * k + 3 >= SA) && (4 * k < SA + NA);
+    float run[LA];
+#pragma unroll
+    for (int ph = 0; ph < PH; ph++) {
+#pragma unroll
+        for (int r = 0; r < RPP; r++) {
+            const int tt = r * RPR + tl;
+            const int t = ph * OPP + tt;
+            if (serve && tt < OPP && t < 64) {
+                const float *q = base + __float_as_int(tab[t]) + 4 * k;
+                const float w00 = tab[64 + t], w01 = tab[128 + t], w10 = tab[192 + t], w11 = tab[256 + t];
+                const float4 a = *reinterpret_cast<const float4 *>(q);
+                const float4 b = *reinterpret_cast<const float4 *>(q + dx);
+                const float4 c = *reinterpret_cast<const float4 *>(q + dz);
+                const float4 d = *reinterpret_cast<const float4 *>(q + dz + dx);
+                float4 v;
+                v.x = a.x * w00 + b.x * w01 + c.x * w10 + d.x * w11;
+                v.y = a.y * w00 + b.y * w01 + c.y * w10 + d.y * w11;
+                v.z = a.z * w00 + b.z * w01 + c.z * w10 + d.z * w11;
+                v.w = a.w * w00 + b.w * w01 + c.w * w10 + d.w * w11;
+                *reinterpret_cast<float4 *>(stage + tt * RS + 4 * k) = v;
+            }
+        }
+        wave_sync();
+        if (lane / OPP == ph) {
+            const float *mine = stage + (lane - ph * OPP) * RS;
+#pragma unroll
+            for (int qd = 0; qd < CH; qd++) {
+                if (4 * qd + 3 < SA || 4 * qd >= SA + NA) continue;
+                const float4 v = *reinterpret_cast<const float4 *>(mine + 4 * qd);
+                run[4 * qd] = v.x;
+                run[4 * qd + 1] = v.y;
+                run[4 * qd + 2] = v.z;
+                run[4 * qd + 3] = v.w;
+            }
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int j = 0; j < NA; j++) out[j] = f(j, run[j + SA]);
+}
+
 // Row (x) pass through LDS for a block whose thread tx owns column x' = tx:
 // out[i] = sum_j buf[tx + j][i] * kr[2R-j].  buf holds (nthr + 2R) columns of
 // TS+4 floats with R zero columns on each side.  One barrier; the caller
@@ -314,7 +403,7 @@ __device__ __forceinline__ void zero_lds(float *lds, int n, int tx, int nthr)
 // grid: (slabs, V*D); block: nthr >= nx threads, thread tx owns column x' = tx.
 // PROJECT: partial projection of chunk d written to part[d][v][y][x]
 // (the sinogram itself when D == 1).  Reference emission-lib/_et.cpp:218-281.
-template <int TS, int R, bool HAS_MU>
+template <int TS, int R, bool HAS_MU, bool COOP>
 __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
 {
     constexpr int K = 2 * R + 1;
@@ -323,6 +412,16 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
     extern __shared__ float lds[];
     const SweepCtx c = sweep_ctx<TS, R>(a, true);
     zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
+    // Cooperative staging: with a PSF, a wave stages rows in its own slice of
+    // the row-pass buffer that is idle at that moment (buffer z&1 before the
+    // row pass of plane z, buffer (z+1)&1 after it); without one, after the
+    // sample tables.
+    constexpr int L = TS + 4;
+    constexpr int OWN = 64 * L;
+    constexpr int OPA = R > 0 ? coop_owners(LA, OWN) : 64, OPM = R > 0 ? coop_owners(TS, OWN) : 64;
+    const int wv = c.tx >> 6;
+    float *tab = lds + (R > 0 ? 2 * c.bufsz : 0) + wv * ET_COOP_TAB;
+    float *own0 = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
 
     float A[TS], acc[TS];
 #pragma unroll
@@ -345,8 +444,16 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
         float h[TS];
         {
             float yc[TS];
-            resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
-                                               [](int, float v) { return v; }, yc);
+            if constexpr (COOP) {
+                float rows[NA];
+                coop_rows<NA, LA, SA, OPA>(a.act + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab,
+                                           own0 + (R > 0 ? (z & 1) * c.bufsz : 0),
+                                           [](int, float v) { return v; }, rows);
+                column_pass<TS, R>(rows, kz + K, yc);
+            } else {
+                resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
+                                                   [](int, float v) { return v; }, yc);
+            }
             if constexpr (R == 0) {
 #pragma unroll
                 for (int i = 0; i < TS; i++) h[i] = yc[i];
@@ -359,10 +466,18 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
             }
         }
         if constexpr (HAS_MU) {
-            bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, [&](int i, float m) {
+            auto att = [&](int i, float m) {
                 A[i] += m;
                 acc[i] = fmaf(h[i], exp_neg(A[i]), acc[i]);
-            });
+                return 0.f;
+            };
+            if constexpr (COOP) {
+                float dummy[TS];
+                coop_rows<TS, TS, 0, OPM>(a.mu + (a.gy + c.y0), c.dx, c.dz, s, tab,
+                                          own0 + (R > 0 ? ((z + 1) & 1) * c.bufsz : 0), att, dummy);
+            } else {
+                bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, att);
+            }
         } else {
 #pragma unroll
             for (int i = 0; i < TS; i++) acc[i] += h[i];
@@ -388,7 +503,7 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
 
 // BPU: u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ) for the slab rows,
 // reference emission-lib/_et.cpp:441-471 (line backprojection, then PSF).
-template <int TS, int R, bool HAS_MU>
+template <int TS, int R, bool HAS_MU, bool COOP>
 __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
 {
     constexpr int K = 2 * R + 1;
@@ -397,6 +512,13 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
     extern __shared__ float lds[];
     const SweepCtx c = sweep_ctx<TS, R>(a, true);
     zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
+    // Cooperative staging in the wave's own slice of row-pass buffer z&1,
+    // idle until the row pass of plane z (see k_project).
+    constexpr int L = TS + 4;
+    constexpr int OPA = R > 0 ? coop_owners(LA, 64 * L) : 64;
+    const int wv = c.tx >> 6;
+    float *tab = lds + (R > 0 ? 2 * c.bufsz : 0) + wv * ET_COOP_TAB;
+    float *own0 = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
 
     float sv[NA], A[NA];
 #pragma unroll
@@ -412,12 +534,19 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         float yc[TS];
         if constexpr (HAS_MU) {
-            resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
-                                               [&](int j, float m) {
-                                                   A[j] += m;
-                                                   return sv[j] * exp_neg(A[j]);
-                                               },
-                                               yc);
+            auto wfun = [&](int j, float m) {
+                A[j] += m;
+                return sv[j] * exp_neg(A[j]);
+            };
+            if constexpr (COOP) {
+                float rows[NA];
+                coop_rows<NA, LA, SA, OPA>(a.mu + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab,
+                                           own0 + (R > 0 ? (z & 1) * c.bufsz : 0), wfun, rows);
+                column_pass<TS, R>(rows, kz + K, yc);
+            } else {
+                resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K, wfun,
+                                                   yc);
+            }
         } else if constexpr (R == 0) {
 #pragma unroll
             for (int i = 0; i < TS; i++) yc[i] = sv[i];
@@ -576,13 +705,14 @@ __global__ void __launch_bounds__(512) k_musum(EtSweepArgs a)
 // pull, accumulation in view order (emission-lib/_et.cpp:474-498).  The fast
 // path runs with zero padding only; a non-zero background takes the general path.
 __device__ __forceinline__ void pull_ref_view(const EtPullArgs &a, const float *rec, const float *uv,
-                                              int x, int z, int y0, float *acc)
+                                              int x, int z, int y0, float *tab, float *stage, float *acc)
 {
     const float qx = pos2(rec[12], (float)x, rec[14], (float)z, rec[15]);
     const float qz = pos2(rec[20], (float)x, rec[22], (float)z, rec[23]);
     const int dx = a.yu, dz = a.yu * a.xp;
     const Samp s = make_samp(qx, qz, a.nx, a.nz, dx, dz);
-    bilinear_map<T, T, 0>(uv + s.off + y0, dx, dz, s, [&](int i, float v) { acc[i] += v; });
+    float dummy[T];
+    coop_rows<T, T, 0, 64>(uv + y0, dx, dz, s, tab, stage, [&](int i, float v) { acc[i] += v; return 0.f; }, dummy);
 }
 
 // Exact adjoint of the forward fast-path pull: gathers from every rotated-frame
@@ -626,13 +756,21 @@ __device__ __forceinline__ void pull_adj_view(const EtPullArgs &a, const float *
     }
 }
 
+// Thread = output column (x, z) with T axial rows; the reference variant keeps
+// every wave converged (columns past the grid sample a clamped column and
+// store nothing) because its sampler is wave-cooperative.
 template <bool ADJ, int EPI>
 __global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.nx * a.nz) return;
+    constexpr int SCR = ET_COOP_TAB + 64 * (T + 4);
+    __shared__ float scr[ADJ ? 4 : 4 * SCR];
+    const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool cin = c0 < a.nx * a.nz;
+    if (ADJ && !cin) return;
+    const int c = cin ? c0 : a.nx * a.nz - 1;
     const int x = c % a.nx, z = c / a.nx;
     const int y0 = blockIdx.y * T;
+    float *tab = scr + (threadIdx.x >> 6) * SCR;
     float acc[T];
 #pragma unroll
     for (int i = 0; i < T; i++) acc[i] = 0.f;
@@ -642,8 +780,9 @@ __global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
         if (ADJ)
             pull_adj_view(a, rec, uv, x, z, y0, acc);
         else
-            pull_ref_view(a, rec, uv, x, z, y0, acc);
+            pull_ref_view(a, rec, uv, x, z, y0, tab, tab + ET_COOP_TAB, acc);
     }
+    if (!cin) return;
     if constexpr (EPI == ET_EPI_ABI_STORE) {
 #pragma unroll
         for (int i = 0; i < T; i++) {
@@ -1055,29 +1194,69 @@ static int slab_rows(int mode, int R)
 
 int et_sweep_slab_rows(int mode, int R) { return slab_rows(mode, R); }
 
-static size_t sweep_lds_bytes(int R, int TS, int nthr)
+static size_t sweep_lds_bytes(int R, int TS, int nthr, bool coop)
 {
-    return R > 0 ? (size_t)2 * (nthr + 2 * R) * (TS + 4) * sizeof(float) : 0;
+    const int R4 = (R + 3) & ~3;
+    size_t f = R > 0 ? (size_t)2 * (nthr + 2 * R) * (TS + 4) : 0;
+    if (coop) f += (size_t)(nthr / 64) * (ET_COOP_TAB + (R > 0 ? 0 : 64 * (TS + 2 * R4 + 4)));
+    return f * sizeof(float);
+}
+
+static size_t lds_limit()
+{
+    static size_t lim = 0;
+    if (!lim) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            lim = (size_t)v;
+        else
+            lim = 65536;
+    }
+    return lim;
+}
+
+template <class Kern>
+static void allow_lds(Kern k, size_t bytes)
+{
+    if (bytes > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 template <int TS, int R>
 static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st)
 {
     const int nthr = ((a.nx + 63) / 64) * 64;
-    const size_t lds = sweep_lds_bytes(R, TS, nthr);
     const int slabs = (a.ny + TS - 1) / TS;
+    const bool coop = sweep_lds_bytes(R, TS, nthr, true) <= lds_limit();
+    const bool uses = mode == ET_MODE_PROJECT || (mode == ET_MODE_BPU && has_mu);
+    const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses);
     if (mode == ET_MODE_PROJECT) {
         dim3 g(slabs, a.V * a.D);
-        if (has_mu)
-            hipLaunchKernelGGL((k_project<TS, R, true>), g, dim3(nthr), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_project<TS, R, false>), g, dim3(nthr), lds, st, a);
+#define ET_LP(mu, cp)                                                   \
+    {                                                                   \
+        allow_lds(k_project<TS, R, mu, cp>, lds);                       \
+        hipLaunchKernelGGL((k_project<TS, R, mu, cp>), g, dim3(nthr), lds, st, a); \
+    }
+        if (has_mu) {
+            if (coop) ET_LP(true, true) else ET_LP(true, false)
+        } else {
+            if (coop) ET_LP(false, true) else ET_LP(false, false)
+        }
+#undef ET_LP
     } else if (mode == ET_MODE_BPU) {
         dim3 g(slabs, a.V * a.D);
-        if (has_mu)
-            hipLaunchKernelGGL((k_bpu<TS, R, true>), g, dim3(nthr), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_bpu<TS, R, false>), g, dim3(nthr), lds, st, a);
+#define ET_LB(mu, cp)                                               \
+    {                                                               \
+        allow_lds(k_bpu<TS, R, mu, cp>, lds);                       \
+        hipLaunchKernelGGL((k_bpu<TS, R, mu, cp>), g, dim3(nthr), lds, st, a); \
+    }
+        if (has_mu) {
+            if (coop) ET_LB(true, true) else ET_LB(true, false)
+        } else {
+            ET_LB(false, false)
+        }
+#undef ET_LB
     } else if (mode == ET_MODE_GRAD_REF) {
         hipLaunchKernelGGL((k_grad_ref<TS, R>), dim3(slabs, a.V), dim3(nthr), lds, st, a);
     } else if (mode == ET_MODE_GRAD_MU) {
