@@ -1,6 +1,6 @@
 """Benchmark: OSEM iterations/s for SPECT at 128^3 x 120 views, depth-dependent
-PSF 11x11 and attenuation, 8 ordered subsets (the config-2 workload), on one
-MI355X with every input resident in HBM.
+PSF 11x11 and attenuation, 8 ordered subsets of 15 views, on one MI355X with
+every input resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
@@ -29,7 +29,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--workload", default="spect128")
     return ap.parse_args()
 
 
@@ -69,7 +69,7 @@ def main():
     from niftyrec_amd import api, phantoms
 
     w = phantoms.workload(args.workload)
-    S = w["subsets"] if args.workload == "c2" else 1
+    S = w["subsets"] if args.workload == "spect128" else 1
     act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
     nz, ny, nx = act.shape
     V = len(cams)
@@ -101,7 +101,7 @@ def main():
     its = args.steps / elapsed
     n = nx * ny * nz
     P = nx * ny
-    bytes_iter = V * (16.0 * n + 8.0 * P) + S * 16.0 * n  # BASELINE.md section 2, sensitivity cached
+    bytes_iter = V * (16.0 * n + 8.0 * P) + S * 16.0 * n  # per view: act+mu read, sino read+write; per subset: x, sens read, x write
     # dominant kernel: the category with the largest total time
     cat = max((k for k in prof if prof[k][0] > 0), key=lambda k: prof[k][1])
     launches, tot_ms, tot_bytes = prof[cat]

@@ -192,7 +192,11 @@ int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *
                                  void *stream);
 /* Ordered-subsets EM on the device: subset s holds views s, s+S, s+2S, ...;
  * each subset update is the reference MLEM update restricted to its views.
- * act is updated in place; the sensitivity of every subset is cached in the plan. */
+ * act is updated in place.  The sensitivity of every subset, the attenuation
+ * prefixes and the depth chunking (timed on the first call unless fixed with
+ * et_plan_set_tuning) are cached in the plan, keyed on (subsets, mu pointer,
+ * eps); a caller that rewrites mu in place calls et_plan_set_tuning to drop
+ * the cache. */
 int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, int subsets,
                  int iterations, float epsilon, void *stream);
 /* Tuning: depth chunks per ray (0 = automatic), views per backprojection batch (0 = automatic). */

@@ -224,8 +224,9 @@ struct et_plan {
     // workspaces
     DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol;
     // OSEM caches
-    DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart, osem_order;
-    int osem_S = 0, osem_D = 0;
+    DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart, osem_astart_b, osem_order;
+    int osem_S = 0, osem_D = 0, osem_Dp = 0, osem_Db = 0;
+    bool osem_mu_yf = false;
     const float *osem_mu_key = nullptr;
     float osem_eps = 0.f;
     // tuning
@@ -767,18 +768,20 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
 // MLEM update restricted to views [v0, v0+nv) of the plan with every volume in
 // the YF layout; the reference update of _et_array_interface.cpp:1080-1131.
 // sens_yf must hold the clamped sensitivity of these views.
+// The projection runs in Dp depth chunks and the rotated-frame backprojection
+// in Db (each with its own attenuation prefix, null without mu or for one chunk).
 void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const float *mu_yf, const float *meas,
-                      const float *sens_yf, float eps, int D, const float *astart)
+                      const float *sens_yf, float eps, int Dp, const float *astart_p, int Db, const float *astart_b)
 {
     p->u.ensure((size_t)nv * p->ustride, true, p->st);
-    float *parts = p->part.ensure((size_t)D * nv * p->P, false);
+    float *parts = p->part.ensure((size_t)Dp * nv * p->P, false);
     float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
-    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, D, astart, parts);
+    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, Dp, astart_p, parts);
     {
-        ProfScope ps(p, ET_PROF_OTHER, (D + 2.0) * 4.0 * nv * p->P);
-        check_launch(et_launch_ratio_parts(meas, parts, D, (size_t)nv * p->P, eps, ratio, (et_stream_t)p->st));
+        ProfScope ps(p, ET_PROF_OTHER, (Dp + 2.0) * 4.0 * nv * p->P);
+        check_launch(et_launch_ratio_parts(meas, parts, Dp, (size_t)nv * p->P, eps, ratio, (et_stream_t)p->st));
     }
-    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, D, astart, false, p->u.p);
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, Db, astart_b, false, p->u.p);
     pull_fast(p, recs, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, x_yf, sens_yf);
 }
 
@@ -812,6 +815,73 @@ void em_step_general(et_plan *p, float *x, const float *meas, const float *mu, f
     op_backproject(p, proj, mu, s, false, 1);
     check_launch(et_launch_clamp_min(s, p->nvox, eps, (et_stream_t)p->st));
     check_launch(et_launch_em_update_flat(x, b, s, p->nvox, (et_stream_t)p->st));
+}
+
+// Cached attenuation prefixes of the OSEM subsets (first view index v0), for
+// the projection and the rotated-frame backprojection chunkings.
+const float *osem_astart_p(et_plan *p, int v0)
+{
+    if (!p->osem_mu_yf || p->osem_Dp <= 1) return nullptr;
+    return p->osem_astart.p + (size_t)v0 * p->osem_Dp * p->P;
+}
+
+const float *osem_astart_b(et_plan *p, int v0)
+{
+    if (!p->osem_mu_yf || p->osem_Db <= 1) return nullptr;
+    if (p->osem_Db == p->osem_Dp) return osem_astart_p(p, v0);
+    return p->osem_astart_b.p + (size_t)v0 * p->osem_Db * p->P;
+}
+
+// Depth chunking of the projection (mode ET_MODE_PROJECT) or rotated-frame
+// backprojection (ET_MODE_BPU) sweep for nv views, timed on the plan's stream:
+// candidates 4..32 chunks (at least 4 planes each), three launches each after
+// one warm-up, fastest average kept.  More chunks mean more, shorter blocks
+// (better balance over the CUs) at the cost of more partial sums.  Inputs are
+// the plan's own work buffers; the values computed are discarded.
+int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const float *mu_yf)
+{
+    std::vector<int> cand;
+    for (int D = 4; D <= 32; D *= 2)
+        if (p->nz / D >= 4) cand.push_back(D);
+    if (cand.empty()) return 1;
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), niftyrec_error_kernel);
+    check_hip(hipEventCreate(&e1), niftyrec_error_kernel);
+    const bool prof = p->prof;
+    p->prof = false;
+    const float *act = mu_yf ? mu_yf : p->act_yf.ensure(p->yf_elems, true, p->st);
+    int best = cand[0];
+    float best_ms = 1e30f;
+    for (int D : cand) {
+        const float *as = nullptr;
+        if (mu_yf) {
+            mu_prefix(p, recs, nv, D, mu_yf, true, false);
+            as = p->astart.p;
+        }
+        float *parts = p->part.ensure((size_t)D * nv * p->P, false);
+        p->u.ensure((size_t)nv * p->ustride, true, p->st);
+        auto run = [&]() {
+            if (mode == ET_MODE_PROJECT)
+                project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts);
+            else
+                bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p);
+        };
+        run();
+        check_hip(hipEventRecord(e0, p->st), niftyrec_error_kernel);
+        for (int r = 0; r < 3; r++) run();
+        check_hip(hipEventRecord(e1, p->st), niftyrec_error_kernel);
+        check_hip(hipEventSynchronize(e1), niftyrec_error_kernel);
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), niftyrec_error_kernel);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = D;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->prof = prof;
+    return best;
 }
 
 void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, int iters, float eps)
@@ -850,17 +920,27 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
         check_hip(hipMemcpyAsync(p->osem_order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice,
                                  p->st),
                   niftyrec_error_transfergpu);
+        // chunking: fixed by et_plan_set_tuning, else timed on the first subset
+        p->osem_S = 0;  // invalid until the rebuild completes
+        int Dp = D, Db = D;
+        if (p->tune_D <= 0) {
+            Dp = tune_depth_chunks(p, ET_MODE_PROJECT, p->osem_recs.p, start[1], mu_yf);
+            Db = tune_depth_chunks(p, ET_MODE_BPU, p->osem_recs.p, start[1], mu_yf);
+        }
+        p->osem_Dp = Dp;
+        p->osem_Db = Db;
+        p->osem_mu_yf = mu_yf != nullptr;
         p->sens.ensure((size_t)S * p->yf_elems, true, p->st);
-        if (mu_yf && D > 1) p->osem_astart.ensure((size_t)p->V * D * p->P, false);
+        if (mu_yf && Dp > 1) p->osem_astart.ensure((size_t)p->V * Dp * p->P, false);
+        if (mu_yf && Db > 1 && Db != Dp) p->osem_astart_b.ensure((size_t)p->V * Db * p->P, false);
         for (int s = 0; s < S; s++) {
             const int nv = start[s + 1] - start[s];
             const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
-            float *as = nullptr;
-            if (mu_yf && D > 1) {
-                as = p->osem_astart.p + (size_t)start[s] * D * p->P;
-                mu_prefix(p, r, nv, D, mu_yf, true, false, as);
-            }
-            em_sensitivity(p, r, nv, mu_yf, eps, D, as, p->sens.p + (size_t)s * p->yf_elems);
+            if (mu_yf && Dp > 1)
+                mu_prefix(p, r, nv, Dp, mu_yf, true, false, p->osem_astart.p + (size_t)start[s] * Dp * p->P);
+            if (mu_yf && Db > 1 && Db != Dp)
+                mu_prefix(p, r, nv, Db, mu_yf, true, false, p->osem_astart_b.p + (size_t)start[s] * Db * p->P);
+            em_sensitivity(p, r, nv, mu_yf, eps, Db, osem_astart_b(p, start[s]), p->sens.p + (size_t)s * p->yf_elems);
         }
         check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
         p->osem_S = S;
@@ -880,9 +960,9 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
         for (int s = 0; s < S; s++) {
             const int nv = start[s + 1] - start[s];
             const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
-            const float *as = (mu_yf && D > 1) ? p->osem_astart.p + (size_t)start[s] * D * p->P : nullptr;
             em_subset_update(p, r, nv, p->x_yf.p, mu_yf, p->osem_meas.p + (size_t)start[s] * p->P,
-                             p->sens.p + (size_t)s * p->yf_elems, eps, D, as);
+                             p->sens.p + (size_t)s * p->yf_elems, eps, p->osem_Dp, osem_astart_p(p, start[s]),
+                             p->osem_Db, osem_astart_b(p, start[s]));
         }
     ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
     check_launch(et_launch_unpack_yf(p->x_yf.p, x, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));

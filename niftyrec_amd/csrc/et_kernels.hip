@@ -266,8 +266,9 @@ __device__ __forceinline__ void coop_rows(const float *__restrict__ base, int dx
 
 // Row (x) pass through LDS for a block whose thread tx owns column x' = tx:
 // out[i] = sum_j buf[tx + j][i] * kr[2R-j].  buf holds (nthr + 2R) columns of
-// TS+4 floats with R zero columns on each side.  One barrier; the caller
-// alternates two buffers so the next write cannot race these reads.
+// TS+4 floats with R zero columns on each side.  One barrier between the
+// write and the reads; the caller keeps the next write from racing these reads
+// (a block barrier before it, or a second buffer).
 template <int TS, int R>
 __device__ __forceinline__ void row_pass(float *buf, int tx, const float *yc, const float *__restrict__ kr,
                                          float *out)
@@ -365,16 +366,32 @@ struct SweepCtx {
     FastView fv;
 };
 
+// XCD-aware block order.  The dispatcher deals workgroups to the 8 XCDs
+// round-robin by linear id (x fastest), and each XCD has its own L2.  For a
+// grid of S fast x N slow blocks with N a multiple of 8, block i is given
+// fast index (i/8) % S and slow index ((i/8) / S) * 8 + i % 8: each XCD then
+// runs all S fast blocks of one slow index back to back, so the cache lines
+// they share (neighbouring axial slabs, halo rows) are fetched once per XCD.
+// Otherwise the launch order (fast0, slow0) is kept.  Returns (fast, slow).
+__device__ __forceinline__ int2 xcd_order(int S, int N, int fast0, int slow0)
+{
+    if ((N & 7) != 0) return make_int2(fast0, slow0);
+    const int i = blockIdx.x + gridDim.x * blockIdx.y;
+    const int seq = i >> 3;
+    return make_int2(seq % S, (seq / S) * 8 + (i & 7));
+}
+
 template <int TS, int R>
 __device__ __forceinline__ SweepCtx sweep_ctx(const EtSweepArgs &a, bool chunked)
 {
     SweepCtx c;
     c.tx = threadIdx.x;
     c.nthr = blockDim.x;
-    const int vd = blockIdx.y;
+    const int2 fs = xcd_order(gridDim.x, gridDim.y, blockIdx.x, blockIdx.y);
+    const int vd = fs.y;
     c.v = chunked ? vd / a.D : vd;
     c.d = chunked ? vd - c.v * a.D : 0;
-    c.y0 = blockIdx.x * TS;
+    c.y0 = fs.x * TS;
     c.zb = chunked ? c.d * a.zlen : 0;
     c.ze = chunked ? min(a.nz, c.zb + a.zlen) : a.nz;
     c.xin = c.tx < a.nx;
@@ -411,17 +428,17 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
     constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
     extern __shared__ float lds[];
     const SweepCtx c = sweep_ctx<TS, R>(a, true);
-    zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
-    // Cooperative staging: with a PSF, a wave stages rows in its own slice of
-    // the row-pass buffer that is idle at that moment (buffer z&1 before the
-    // row pass of plane z, buffer (z+1)&1 after it); without one, after the
-    // sample tables.
+    zero_lds<R>(lds, c.bufsz, c.tx, c.nthr);
+    // One row-pass buffer.  Each plane starts with a block barrier (the row
+    // pass of the previous plane has been read by every wave); until its own
+    // row-pass write, a wave stages cooperative rows in its own slice of the
+    // buffer.  Without a PSF the stages follow the sample tables.
     constexpr int L = TS + 4;
     constexpr int OWN = 64 * L;
     constexpr int OPA = R > 0 ? coop_owners(LA, OWN) : 64, OPM = R > 0 ? coop_owners(TS, OWN) : 64;
     const int wv = c.tx >> 6;
-    float *tab = lds + (R > 0 ? 2 * c.bufsz : 0) + wv * ET_COOP_TAB;
-    float *own0 = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
+    float *tab = lds + (R > 0 ? c.bufsz : 0) + wv * ET_COOP_TAB;
+    float *stage = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
 
     float A[TS], acc[TS];
 #pragma unroll
@@ -441,47 +458,44 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
-        float h[TS];
-        {
-            float yc[TS];
-            if constexpr (COOP) {
-                float rows[NA];
-                coop_rows<NA, LA, SA, OPA>(a.act + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab,
-                                           own0 + (R > 0 ? (z & 1) * c.bufsz : 0),
-                                           [](int, float v) { return v; }, rows);
-                column_pass<TS, R>(rows, kz + K, yc);
-            } else {
-                resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
-                                                   [](int, float v) { return v; }, yc);
-            }
-            if constexpr (R == 0) {
-#pragma unroll
-                for (int i = 0; i < TS; i++) h[i] = yc[i];
-            } else {
-                if (!c.xin) {
-#pragma unroll
-                    for (int i = 0; i < TS; i++) yc[i] = 0.f;
-                }
-                row_pass<TS, R>(lds + (z & 1) * c.bufsz, c.tx, yc, kz, h);
-            }
+        if (R > 0 && z > c.zb) __syncthreads();
+        float yc[TS];
+        if constexpr (COOP) {
+            float rows[NA];
+            coop_rows<NA, LA, SA, OPA>(a.act + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab, stage,
+                                       [](int, float v) { return v; }, rows);
+            column_pass<TS, R>(rows, kz + K, yc);
+        } else {
+            resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
+                                               [](int, float v) { return v; }, yc);
         }
         if constexpr (HAS_MU) {
-            auto att = [&](int i, float m) {
+            // attenuation up to and including plane z (used after the row pass)
+            auto cum = [&](int i, float m) {
                 A[i] += m;
-                acc[i] = fmaf(h[i], exp_neg(A[i]), acc[i]);
                 return 0.f;
             };
             if constexpr (COOP) {
                 float dummy[TS];
-                coop_rows<TS, TS, 0, OPM>(a.mu + (a.gy + c.y0), c.dx, c.dz, s, tab,
-                                          own0 + (R > 0 ? ((z + 1) & 1) * c.bufsz : 0), att, dummy);
+                coop_rows<TS, TS, 0, OPM>(a.mu + (a.gy + c.y0), c.dx, c.dz, s, tab, stage, cum, dummy);
             } else {
-                bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, att);
+                bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s,
+                                        [&](int i, float m) { A[i] += m; });
             }
-        } else {
-#pragma unroll
-            for (int i = 0; i < TS; i++) acc[i] += h[i];
         }
+        float h[TS];
+        if constexpr (R == 0) {
+#pragma unroll
+            for (int i = 0; i < TS; i++) h[i] = yc[i];
+        } else {
+            if (!c.xin) {
+#pragma unroll
+                for (int i = 0; i < TS; i++) yc[i] = 0.f;
+            }
+            row_pass<TS, R>(lds, c.tx, yc, kz, h);
+        }
+#pragma unroll
+        for (int i = 0; i < TS; i++) acc[i] = HAS_MU ? fmaf(h[i], exp_neg(A[i]), acc[i]) : acc[i] + h[i];
     }
     if (c.d == a.D - 1) {
 #pragma unroll
@@ -511,14 +525,14 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
     constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
     extern __shared__ float lds[];
     const SweepCtx c = sweep_ctx<TS, R>(a, true);
-    zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
-    // Cooperative staging in the wave's own slice of row-pass buffer z&1,
-    // idle until the row pass of plane z (see k_project).
+    zero_lds<R>(lds, c.bufsz, c.tx, c.nthr);
+    // One row-pass buffer with cooperative staging in the wave's own slice of
+    // it, as in k_project.
     constexpr int L = TS + 4;
     constexpr int OPA = R > 0 ? coop_owners(LA, 64 * L) : 64;
     const int wv = c.tx >> 6;
-    float *tab = lds + (R > 0 ? 2 * c.bufsz : 0) + wv * ET_COOP_TAB;
-    float *own0 = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
+    float *tab = lds + (R > 0 ? c.bufsz : 0) + wv * ET_COOP_TAB;
+    float *stage = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
 
     float sv[NA], A[NA];
 #pragma unroll
@@ -532,6 +546,7 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
+        if (R > 0 && z > c.zb) __syncthreads();
         float yc[TS];
         if constexpr (HAS_MU) {
             auto wfun = [&](int j, float m) {
@@ -540,8 +555,7 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
             };
             if constexpr (COOP) {
                 float rows[NA];
-                coop_rows<NA, LA, SA, OPA>(a.mu + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab,
-                                           own0 + (R > 0 ? (z & 1) * c.bufsz : 0), wfun, rows);
+                coop_rows<NA, LA, SA, OPA>(a.mu + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab, stage, wfun, rows);
                 column_pass<TS, R>(rows, kz + K, yc);
             } else {
                 resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K, wfun,
@@ -564,7 +578,7 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
 #pragma unroll
             for (int i = 0; i < TS; i++) out[i] = yc[i];
         } else {
-            row_pass<TS, R>(lds + (z & 1) * c.bufsz, c.tx, yc, kz, out);
+            row_pass<TS, R>(lds, c.tx, yc, kz, out);
         }
         if (c.xin) store_u<TS>(a, c.v, c.tx, z, c.y0, out);
     }
@@ -756,6 +770,62 @@ __device__ __forceinline__ void pull_adj_view(const EtPullArgs &a, const float *
     }
 }
 
+// Coalesced YF epilogue of the reference pull: the wave's 64 columns of T
+// rows go through its LDS stage so that 4 consecutive lanes handle the 4
+// aligned float4 chunks of one column run (YF runs start at multiples of 16
+// floats).  Padding rows (y >= ny) keep their content, so the zero border of
+// the YF layout survives.  Every lane of the wave must call it.
+template <int EPI>
+__device__ __forceinline__ void yf_epilogue_coop(const EtPullArgs &a, const float *acc, float *stage,
+                                                 int cbase, int y0)
+{
+    static_assert(T == 16, "epilogue assumes 16-row runs");
+    constexpr int RS = T + 4;
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int q = 0; q < T / 4; q++)
+        *reinterpret_cast<float4 *>(stage + lane * RS + 4 * q) =
+            make_float4(acc[4 * q], acc[4 * q + 1], acc[4 * q + 2], acc[4 * q + 3]);
+    wave_sync();
+    const int k = lane & 3;
+    const int ncol = a.nx * a.nz;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int t = 16 * r + (lane >> 2);
+        const int c = cbase + t;
+        if (c < ncol) {
+            const int x = c % a.nx, z = c / a.nx;
+            const size_t base = (a.gy + y0 + 4 * k) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
+            const float4 b = *reinterpret_cast<const float4 *>(stage + t * RS + 4 * k);
+            const float bv[4] = {b.x, b.y, b.z, b.w};
+            float *dst = EPI == ET_EPI_OSEM ? a.x : a.out;
+            float4 *d4 = reinterpret_cast<float4 *>(dst + base);
+            const float4 o = (EPI == ET_EPI_OSEM || a.accumulate) ? *d4 : make_float4(0.f, 0.f, 0.f, 0.f);
+            const float ov[4] = {o.x, o.y, o.z, o.w};
+            float sv[4] = {1.f, 1.f, 1.f, 1.f};
+            if constexpr (EPI == ET_EPI_OSEM) {
+                const float4 s4 = *reinterpret_cast<const float4 *>(a.sens + base);
+                sv[0] = s4.x; sv[1] = s4.y; sv[2] = s4.z; sv[3] = s4.w;
+            }
+            float nv[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const bool live = y0 + 4 * k + e < a.ny;
+                float val;
+                if constexpr (EPI == ET_EPI_OSEM) {
+                    const float bb = bv[e] < 0.f ? 0.f : bv[e];
+                    val = ov[e] * (bb / sv[e]);
+                } else {
+                    val = a.accumulate ? ov[e] + bv[e] : bv[e];
+                    if (a.truncate && val < 0.f) val = 0.f;
+                }
+                nv[e] = live ? val : ov[e];
+            }
+            *d4 = make_float4(nv[0], nv[1], nv[2], nv[3]);
+        }
+    }
+}
+
 // Thread = output column (x, z) with T axial rows; the reference variant keeps
 // every wave converged (columns past the grid sample a clamped column and
 // store nothing) because its sampler is wave-cooperative.
@@ -764,12 +834,15 @@ __global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
 {
     constexpr int SCR = ET_COOP_TAB + 64 * (T + 4);
     __shared__ float scr[ADJ ? 4 : 4 * SCR];
-    const int c0 = blockIdx.x * blockDim.x + threadIdx.x;
+    // fast index: axial slab (neighbouring slabs share cache lines of u)
+    const int2 fs = xcd_order(gridDim.y, gridDim.x, blockIdx.y, blockIdx.x);
+    const int cbase = fs.y * blockDim.x;
+    const int c0 = cbase + threadIdx.x;
     const bool cin = c0 < a.nx * a.nz;
     if (ADJ && !cin) return;
     const int c = cin ? c0 : a.nx * a.nz - 1;
     const int x = c % a.nx, z = c / a.nx;
-    const int y0 = blockIdx.y * T;
+    const int y0 = fs.x * T;
     float *tab = scr + (threadIdx.x >> 6) * SCR;
     float acc[T];
 #pragma unroll
@@ -782,7 +855,7 @@ __global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
         else
             pull_ref_view(a, rec, uv, x, z, y0, tab, tab + ET_COOP_TAB, acc);
     }
-    if (!cin) return;
+    if (!cin && (ADJ || EPI == ET_EPI_ABI_STORE)) return;
     if constexpr (EPI == ET_EPI_ABI_STORE) {
 #pragma unroll
         for (int i = 0; i < T; i++) {
@@ -794,6 +867,8 @@ __global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
                 *o = val;
             }
         }
+    } else if constexpr (!ADJ) {
+        yf_epilogue_coop<EPI>(a, acc, tab + ET_COOP_TAB, cbase + (threadIdx.x & ~63), y0);
     } else if constexpr (EPI == ET_EPI_YF_STORE) {
         float *o = a.out + (a.gy + y0) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
 #pragma unroll
@@ -1194,10 +1269,13 @@ static int slab_rows(int mode, int R)
 
 int et_sweep_slab_rows(int mode, int R) { return slab_rows(mode, R); }
 
-static size_t sweep_lds_bytes(int R, int TS, int nthr, bool coop)
+// LDS of a sweep launch: row-pass buffers (one for the projector and the
+// rotated-frame backprojection, two for the gradient sweeps) and, for the
+// cooperative variants, the per-wave sample tables (plus stages without a PSF).
+static size_t sweep_lds_bytes(int R, int TS, int nthr, bool coop, int nbuf)
 {
     const int R4 = (R + 3) & ~3;
-    size_t f = R > 0 ? (size_t)2 * (nthr + 2 * R) * (TS + 4) : 0;
+    size_t f = R > 0 ? (size_t)nbuf * (nthr + 2 * R) * (TS + 4) : 0;
     if (coop) f += (size_t)(nthr / 64) * (ET_COOP_TAB + (R > 0 ? 0 : 64 * (TS + 2 * R4 + 4)));
     return f * sizeof(float);
 }
@@ -1228,9 +1306,10 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
 {
     const int nthr = ((a.nx + 63) / 64) * 64;
     const int slabs = (a.ny + TS - 1) / TS;
-    const bool coop = sweep_lds_bytes(R, TS, nthr, true) <= lds_limit();
+    const bool single = mode == ET_MODE_PROJECT || mode == ET_MODE_BPU;
+    const bool coop = sweep_lds_bytes(R, TS, nthr, true, 1) <= lds_limit();
     const bool uses = mode == ET_MODE_PROJECT || (mode == ET_MODE_BPU && has_mu);
-    const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses);
+    const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses, single ? 1 : 2);
     if (mode == ET_MODE_PROJECT) {
         dim3 g(slabs, a.V * a.D);
 #define ET_LP(mu, cp)                                                   \

@@ -66,22 +66,22 @@ def cameras(n_views, first=0.0, span=2 * np.pi):
 
 
 def workload(name, seed=0):
-    """Inputs of a named configuration.
+    """Inputs of a named workload.
 
-    c1: 64^3, 64 views, PSF 7 (sigma 0.5 -> 2.0), mu 0.15 in the body, 1e6 counts
-    c2: 128^3, 120 views, PSF 11 (0.6 -> 3.0), mu 0.15 + lung-like 0.05, 1e7 counts,
-        8 ordered subsets
-    c5: 256^3, 180 views, PSF 21 (0.8 -> 4.5), mu 0.15/0.05/0.30, 5e7 counts
-    small: 32^3, 16 views, PSF 5, for quick checks
+    spect64:  64^3, 64 views, PSF 7 (sigma 0.5 -> 2.0), mu 0.15 in the body, 1e6 counts
+    spect128: 128^3, 120 views, PSF 11 (0.6 -> 3.0), mu 0.15 + lung-like 0.05, 1e7 counts,
+              8 ordered subsets
+    spect256: 256^3, 180 views, PSF 21 (0.8 -> 4.5), mu 0.15/0.05/0.30, 5e7 counts
+    small:    32^3, 16 views, PSF 5, for quick checks
     """
-    if name == "c1":
+    if name == "spect64":
         shape, V, K, s0, s1, nl, counts = (64, 64, 64), 64, 7, 0.5, 2.0, 8, 1e6
         act, mu = phantom(shape, nl, seed, body_axes=(28, 24, 28))
-    elif name == "c2":
+    elif name == "spect128":
         shape, V, K, s0, s1, nl, counts = (128, 128, 128), 120, 11, 0.6, 3.0, 12, 1e7
         act, mu = phantom(shape, nl, seed, body_axes=(56, 48, 56), lesion_axes=(4, 16),
                           extra_mu=[((0.35, 0.1, 0.0), (16, 24, 20), 0.05)])
-    elif name == "c5":
+    elif name == "spect256":
         shape, V, K, s0, s1, nl, counts = (256, 256, 256), 180, 21, 0.8, 4.5, 24, 5e7
         act, mu = phantom(shape, nl, seed, body_axes=(112, 96, 112), lesion_axes=(8, 32),
                           extra_mu=[((0.35, 0.1, 0.0), (32, 48, 40), 0.05),
@@ -93,7 +93,7 @@ def workload(name, seed=0):
         raise ValueError(name)
     psf = gaussian_psf(shape[0], K, s0, s1)
     return {"activity": act, "attenuation": mu, "psf": psf, "cameras": cameras(V), "counts": counts,
-            "subsets": 8 if name in ("c2",) else 1}
+            "subsets": 8 if name == "spect128" else 1}
 
 
 def poisson_data(noiseless, counts, seed=0):

@@ -171,13 +171,13 @@ def test_rotate(lib):
         _report("rotate order %d" % order, got, orc.resample(img, orc.rotation_matrix(ang, ctr, order), 0.25))
 
 
-def test_c1_full_size(lib):
-    """Full config-1 shapes: 64^3, 64 views, PSF 7, attenuation."""
-    w = phantoms.workload("c1")
+def test_spect64_full_size(lib):
+    """64^3 volume, 64 views, PSF 7, attenuation."""
+    w = phantoms.workload("spect64")
     act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
     st, got = refabi.project(lib, act, mu, cams, psf)
     assert st == 0
-    _report("c1 project", got, orc.project(act, mu, cams, psf), orc.project(act, mu, cams, psf, dtype=np.float64))
+    _report("spect64 project", got, orc.project(act, mu, cams, psf), orc.project(act, mu, cams, psf, dtype=np.float64))
     st, bp = refabi.backproject(lib, got, mu, cams, psf)
     assert st == 0
-    _report("c1 backproject", bp, orc.backproject(got, mu, cams, psf))
+    _report("spect64 backproject", bp, orc.backproject(got, mu, cams, psf))

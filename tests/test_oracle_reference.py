@@ -1,5 +1,5 @@
 """The single-precision oracle against the live reference CPU build
-(oracle/_ref/lib_et_ref.so) at full configuration shapes; skipped where the
+(oracle/_ref/lib_et_ref.so) at full workload shapes; skipped where the
 reference build is absent.  Bit-exact equality is expected: the oracle keeps
 the reference's order of operations."""
 import os
@@ -18,9 +18,9 @@ def ref():
     return refabi.load_ref()
 
 
-def test_c1_projection_backprojection(ref):
+def test_spect64_projection_backprojection(ref):
     """Config 1 at full size: 64^3, 64 views, PSF 7, attenuation."""
-    w = phantoms.workload("c1")
+    w = phantoms.workload("spect64")
     a, m, p, c = w["activity"], w["attenuation"], w["psf"], w["cameras"]
     st, s_ref = refabi.project(ref, a, m, c, p)
     assert st == 0
@@ -30,9 +30,9 @@ def test_c1_projection_backprojection(ref):
     assert np.array_equal(orc.backproject(s_ref, m, c, p), b_ref)
 
 
-def test_c2_shapes_subset_of_views(ref):
+def test_spect128_shapes_subset_of_views(ref):
     """Config 2 volume and PSF (128^3, PSF 11, attenuation) on 6 of its views."""
-    w = phantoms.workload("c2")
+    w = phantoms.workload("spect128")
     a, m, p = w["activity"], w["attenuation"], w["psf"]
     c = w["cameras"][::20]
     st, s_ref = refabi.project(ref, a, m, c, p)
