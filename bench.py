@@ -33,6 +33,29 @@ def parse():
     return ap.parse_args()
 
 
+# Kernel of each profiled category, as named in the rocprofv3 summaries.
+KERNEL_OF = {"project": "void k_project<", "bp_rotated_frame": "void k_bpu<", "bp_pull": "void k_pull<false, 2>"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "osem_spect128_hbm_traffic.txt")
+
+
+def measured_traffic(cat):
+    """HBM bytes per launch of the category's kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (the subset launches: the row with
+    the most dispatches), or None when the summary lacks it."""
+    prefix = KERNEL_OF.get(cat)
+    if prefix is None or not os.path.exists(TRAFFIC_FILE):
+        return None
+    best = None
+    for ln in open(TRAFFIC_FILE):
+        if not ln.startswith(prefix):
+            continue
+        f = ln.split()
+        n, mb = int(f[-4]), float(f[-1])
+        if best is None or n > best[0]:
+            best = (n, mb)
+    return None if best is None else best[1] * 1e6
+
+
 def cpu_baseline(w, subsets):
     """Reference CPU path (oracle/_ref, single thread as built by the reference)
     on one subset update of the same workload; falls back to the oracle port."""
@@ -126,7 +149,7 @@ def main():
         "gvoxel_views_per_s": V * n / (ms * 1e-3) / 1e9,
         "hbm_frac_of_peak": bytes_iter / (ms * 1e-3) / HBM_PEAK,
         "roofline": {"bound": "hbm", "kernel": cat, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(cat),
                      "avg_launch_ms": tot_ms / launches, "algorithmic_bytes_per_launch": tot_bytes / launches},
         "kernel_ms_per_step": {k: v[1] / args.steps for k, v in prof.items() if v[0]},
     }

@@ -52,19 +52,20 @@ __device__ __forceinline__ float pos3(const float *m, float x, float y, float z)
     return ((m[0] * x + m[1] * y) + m[2] * z) + m[3];
 }
 
-// e^{-A} for A >= 0.  x = -A log2(e) is formed as a rounded product plus its
-// exact residual (fma) and the low part of log2(e); 2^x = 2^n 2^f with
-// n = rint(x), |f| <= 1/2, 2^f from the hardware exp2 (v_exp_f32).
+// e^{-A} for A >= 0.  -A log2(e) = x + r with x the rounded product and r its
+// residual (exact fma residual plus the low part of log2(e)), |r| <= 2^-24 |x|;
+// e^{-A} = 2^x 2^r = 2^x (1 + r ln 2) (the dropped term is below 1e-12
+// relative for A < 60), with 2^x from
+// the hardware exp2 (v_exp_f32, 1 ulp).  Six VALU operations.
 __device__ __forceinline__ float exp_neg(float A)
 {
     const float L2E = 1.44269502162933349609375f;   // float(log2 e)
     const float L2E_LO = 1.925963033500011e-08f;    // log2(e) - L2E
-    float x = -A * L2E;
+    const float LN2 = 0.693147180559945309f;
+    const float x = -A * L2E;
     float r = fmaf(-A, L2E, -x);
     r = fmaf(-A, L2E_LO, r);
-    float n = rintf(x);
-    float f = (x - n) + r;
-    return ldexpf(__builtin_amdgcn_exp2f(f), (int)n);
+    return __builtin_amdgcn_exp2f(x) * fmaf(r, LN2, 1.f);
 }
 
 // Bilinear sample in the (x, z) plane shared by a run of axial rows.

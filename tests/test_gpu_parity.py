@@ -171,6 +171,44 @@ def test_rotate(lib):
         _report("rotate order %d" % order, got, orc.resample(img, orc.rotation_matrix(ang, ctr, order), 0.25))
 
 
+def test_affine(lib):
+    """et_array_affine: resampling through a caller-given matrix (sample
+    position = M (x, y, z, 1), first three rows, row-major)."""
+    rng = np.random.default_rng(12)
+    img = rng.random((14, 10, 18), dtype=np.float32)
+    M = orc.rotation_matrix(rng.uniform(-0.7, 0.7, 3).astype(np.float32), np.array([8.0, 4.5, 6.0], np.float32), 3)
+    M = np.ascontiguousarray(M, np.float32)
+    M[:3, 3] += np.array([0.37, -0.61, 0.25], np.float32)
+    out = np.zeros_like(img)
+    sz = np.array(img.shape[::-1], np.int32)
+    asz = np.array([4, 4], np.int32)
+    F, I = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)
+    fn = lib.et_array_affine
+    fn.argtypes = [F, I, F, I, F, I, ctypes.c_float, ctypes.c_int]
+    st = fn(img.ctypes.data_as(F), sz.ctypes.data_as(I), out.ctypes.data_as(F), sz.ctypes.data_as(I),
+            M.ctypes.data_as(F), asz.ctypes.data_as(I), 0.5, 1)
+    assert st == 0
+    _report("affine", out, orc.resample(img, M, 0.5), orc.resample(img, M, 0.5, dtype=np.float64))
+
+
+def test_convolve(lib):
+    """et_array_convolve: per-plane separated PSF convolution (central row, then
+    central column over the centre value), zero border."""
+    rng = np.random.default_rng(13)
+    img = rng.random((9, 20, 17), dtype=np.float32)
+    psf = phantoms.gaussian_psf(9, 7, 0.6, 2.2)
+    out = np.zeros_like(img)
+    sz = np.array(img.shape[::-1], np.int32)
+    psz = np.array(psf.shape[::-1], np.int32)
+    F, I = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int)
+    fn = lib.et_array_convolve
+    fn.argtypes = [F, I, F, I, F, I, ctypes.c_int]
+    st = fn(img.ctypes.data_as(F), sz.ctypes.data_as(I), out.ctypes.data_as(F), sz.ctypes.data_as(I),
+            np.ascontiguousarray(psf).ctypes.data_as(F), psz.ctypes.data_as(I), 1)
+    assert st == 0
+    _report("convolve", out, orc.convolve_planes(img, psf), orc.convolve_planes(img, psf, dtype=np.float64))
+
+
 def test_spect64_full_size(lib):
     """64^3 volume, 64 views, PSF 7, attenuation."""
     w = phantoms.workload("spect64")

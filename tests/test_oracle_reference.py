@@ -19,7 +19,7 @@ def ref():
 
 
 def test_spect64_projection_backprojection(ref):
-    """Config 1 at full size: 64^3, 64 views, PSF 7, attenuation."""
+    """Full size: 64^3, 64 views, PSF 7, attenuation."""
     w = phantoms.workload("spect64")
     a, m, p, c = w["activity"], w["attenuation"], w["psf"], w["cameras"]
     st, s_ref = refabi.project(ref, a, m, c, p)
@@ -31,7 +31,7 @@ def test_spect64_projection_backprojection(ref):
 
 
 def test_spect128_shapes_subset_of_views(ref):
-    """Config 2 volume and PSF (128^3, PSF 11, attenuation) on 6 of its views."""
+    """128^3 volume, PSF 11, attenuation, on 6 of 120 views."""
     w = phantoms.workload("spect128")
     a, m, p = w["activity"], w["attenuation"], w["psf"]
     c = w["cameras"][::20]
