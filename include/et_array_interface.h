@@ -213,6 +213,28 @@ int et_plan_profile_read(et_plan *plan, int category, int *launches, double *tot
 #define ET_PROF_GRAD 5
 #define ET_PROF_COUNT 6
 
+/* ---- Part 3: batches of plans (no counterpart in the reference) ----------- */
+/* Many independent reconstructions of uneven sizes, angle counts and PSFs run
+ * as one OSEM iteration of grouped launches (plans with the same PSF radius,
+ * volume width class and attenuation presence share each launch), captured
+ * once into a hipGraph and replayed per iteration.  Every plan must take the
+ * fast path (axial cameras, PSF radius <= 10, zero backgrounds) and live on
+ * the same device; plans keep their own caches and work buffers, and must not
+ * be used concurrently with the batch. */
+typedef struct et_batch et_batch;
+
+int et_batch_create(et_batch **batch, et_plan *const *plans, int n_plans);
+int et_batch_destroy(et_batch *batch);
+/* act[i], sino[i] (and mu[i], or mu == NULL / mu[i] == NULL for none): DEVICE
+ * pointers of plan i in the layouts of Part 1; host arrays of n_plans pointers.
+ * Runs `iterations` OSEM iterations of `subsets` ordered subsets on every plan
+ * (the update of et_plan_osem); use_graph = 0 issues the launches directly.
+ * With stream == NULL the call orders after the null stream and returns when
+ * done; otherwise it is asynchronous on `stream`. */
+int et_batch_osem(et_batch *batch, float *const *act, const float *const *sino,
+                  const float *const *mu, int subsets, int iterations, float epsilon,
+                  int use_graph, void *stream);
+
 /* Library version string. */
 const char *et_library_version(void);
 

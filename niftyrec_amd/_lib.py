@@ -52,6 +52,10 @@ SIGNATURES = {
                               c_uint, c_uint, c_float, c_float, c_float, c_int]),
     "et_array_osem": (c_int, [fp, c_uint, c_uint, c_uint, fp, c_int, c_float, c_float, c_uint, c_int, c_int, fp,
                               c_int, fp, c_uint, c_uint, c_float, c_float, c_float, c_int]),
+    "et_batch_create": (c_int, [P(c_void_p), P(c_void_p), c_int]),
+    "et_batch_destroy": (c_int, [c_void_p]),
+    "et_batch_osem": (c_int, [c_void_p, P(c_void_p), P(c_void_p), P(c_void_p), c_int, c_int, c_float, c_int,
+                              c_void_p]),
     "et_plan_create": (c_int, [P(c_void_p), c_int, c_int, c_int, fp, c_int, c_int, fp, c_int, c_float, c_float]),
     "et_plan_destroy": (c_int, [c_void_p]),
     "et_plan_project": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),

@@ -269,3 +269,44 @@ class Plan:
                                                     ctypes.byref(by)), "et_plan_profile_read")
             res[name] = (n.value, ms.value, by.value)
         return res
+
+
+class Batch:
+    """Many plans reconstructed together: one OSEM iteration over every plan as
+    grouped launches captured in one hipGraph (et_batch_osem).  The plans must
+    take the fast path and stay alive while the batch is used."""
+
+    def __init__(self, plans):
+        L = _lib.load()
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p._h.value for p in self.plans])
+        handle = ctypes.c_void_p()
+        _lib.check(L.et_batch_create(ctypes.byref(handle), arr, len(self.plans)), "et_batch_create")
+        self._h = handle
+        self._L = L
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.et_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def osem(self, activities, sinograms, attenuations=None, subsets=1, iterations=1, epsilon=1e-6, use_graph=True):
+        """OSEM in place on every activity tensor (lists in plan order)."""
+        n = len(self.plans)
+        if len(activities) != n or len(sinograms) != n or (attenuations is not None and len(attenuations) != n):
+            raise ValueError("one activity, sinogram (and attenuation) per plan")
+        P = ctypes.c_void_p * n
+        acts = P(*[Plan._ptr(a).value for a in activities])
+        sinos = P(*[Plan._ptr(s).value for s in sinograms])
+        mus = None if attenuations is None else P(*[(Plan._ptr(m).value if m is not None else None)
+                                                    for m in attenuations])
+        st = self._L.et_batch_osem(self._h, acts, sinos, mus, int(subsets), int(iterations), float(epsilon),
+                                   int(bool(use_graph)), Plan._stream())
+        _lib.check(st, "et_batch_osem")
+        return activities

@@ -16,6 +16,8 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "../../include/et_array_interface.h"
@@ -227,6 +229,9 @@ struct et_plan {
     DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart, osem_astart_b, osem_order;
     int osem_S = 0, osem_D = 0, osem_Dp = 0, osem_Db = 0;
     bool osem_mu_yf = false;
+    std::vector<int> osem_start;    // subset s = views osem_start[s] .. osem_start[s+1]-1 (subset order)
+    const float *osem_cur_mu = nullptr;
+    float osem_cur_eps = 0.f;
     const float *osem_mu_key = nullptr;
     float osem_eps = 0.f;
     // tuning
@@ -451,9 +456,9 @@ void mu_prefix(et_plan *p, const float *recs, int nv, int D, const float *mu_yf,
     check_launch(et_launch_chunk_scan(p->csum.p, as, at, nv, D, (int)p->P, (et_stream_t)p->st));
 }
 
-// Fast-path projection of nv views (records recs) into sino [nv][ny][nx].
-void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *sino,
-                  int truncate, int D, const float *astart, float *parts_out = nullptr)
+// Arguments of a projection sweep of nv views into `part` ([D][nv][ny][nx]).
+EtSweepArgs project_args(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *part,
+                         int truncate, int D, const float *astart)
 {
     EtSweepArgs a = base_args(p);
     a.views = recs;
@@ -463,10 +468,19 @@ void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, co
     a.act = act_yf;
     a.mu = mu_yf;
     a.astart = astart;
+    a.part = part;
+    a.truncate = truncate;
+    return a;
+}
+
+// Fast-path projection of nv views (records recs) into sino [nv][ny][nx].
+void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *sino,
+                  int truncate, int D, const float *astart, float *parts_out = nullptr)
+{
     float *part = parts_out;
     if (!part) part = (D == 1) ? sino : p->part.ensure((size_t)D * nv * p->P, false);
-    a.part = part;
-    a.truncate = (D == 1 && !parts_out) ? truncate : 0;
+    const EtSweepArgs a =
+        project_args(p, recs, nv, act_yf, mu_yf, part, (D == 1 && !parts_out) ? truncate : 0, D, astart);
     {
         const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
         ProfScope ps(p, ET_PROF_PROJECT, bytes);
@@ -478,10 +492,10 @@ void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, co
     }
 }
 
-// u for nv views: PSF(s e^{-A}) (flip: correlation kernels).
-void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
-              const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
-              float *u_out)
+// Arguments of a rotated-frame backprojection sweep (see bpu_fast).
+EtSweepArgs bpu_args(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
+                     const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
+                     float *u_out)
 {
     EtSweepArgs a = base_args(p);
     a.views = recs;
@@ -498,13 +512,22 @@ void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src
     a.eps = eps;
     a.u = u_out;
     if (p->K > 0) a.psf = flip ? p->d_sep_flip.p : p->d_sep.p;
+    return a;
+}
+
+// u for nv views: PSF(s e^{-A}) (flip: correlation kernels).
+void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
+              const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
+              float *u_out)
+{
+    const EtSweepArgs a = bpu_args(p, recs, nv, mu_yf, src, sino, meas, rpart, rD, eps, D, astart, flip, u_out);
     const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
     ProfScope ps(p, ET_PROF_BPU, bytes);
     check_launch(et_launch_sweep(ET_MODE_BPU, p->R, mu_yf != nullptr, 0, a, (et_stream_t)p->st));
 }
 
-void pull_fast(et_plan *p, const float *recs, int nv, const float *u, bool adjoint, int epi, bool accumulate,
-               bool truncate, float *out, float *x = nullptr, const float *sens = nullptr)
+EtPullArgs pull_args(et_plan *p, const float *recs, int nv, const float *u, bool adjoint, int epi, bool accumulate,
+                     bool truncate, float *out, float *x, const float *sens)
 {
     EtPullArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -527,6 +550,13 @@ void pull_fast(et_plan *p, const float *recs, int nv, const float *u, bool adjoi
     a.gy = p->gy;
     a.x = x;
     a.sens = sens;
+    return a;
+}
+
+void pull_fast(et_plan *p, const float *recs, int nv, const float *u, bool adjoint, int epi, bool accumulate,
+               bool truncate, float *out, float *x = nullptr, const float *sens = nullptr)
+{
+    const EtPullArgs a = pull_args(p, recs, nv, u, adjoint, epi, accumulate, truncate, out, x, sens);
     const double bytes = (double)nv * 4.0 * p->nvox + (accumulate ? 8.0 : 4.0) * p->nvox;
     ProfScope ps(p, ET_PROF_PULL, bytes);
     check_launch(et_launch_pull(a, (et_stream_t)p->st));
@@ -884,14 +914,13 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
     return best;
 }
 
-void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, int iters, float eps)
+// Fast-path OSEM in three stages: osem_prepare (attenuation packed, subset
+// caches built or reused, measured planes gathered in subset order, x packed
+// into the YF layout), osem_subset (one subset update on the YF iterate) and
+// osem_finish (x unpacked).  The subset layout stays in the plan between them.
+void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int S, float eps)
 {
-    if (S <= 0 || S > p->V) throw EtError{niftyrec_error_parameters};
-    if (!p->fast) {
-        if (S != 1) throw EtError{niftyrec_error_parameters};
-        for (int it = 0; it < iters; it++) em_step_general(p, x, meas, mu, eps);
-        return;
-    }
+    if (S <= 0 || S > p->V || !p->fast) throw EtError{niftyrec_error_parameters};
     const float *mu_yf = nullptr;
     if (mu) {
         pack(p, mu, p->mu_yf);
@@ -956,16 +985,44 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
                                              p->V, p->P, (et_stream_t)p->st));
     }
     pack(p, x, p->x_yf);
-    for (int it = 0; it < iters; it++)
-        for (int s = 0; s < S; s++) {
-            const int nv = start[s + 1] - start[s];
-            const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
-            em_subset_update(p, r, nv, p->x_yf.p, mu_yf, p->osem_meas.p + (size_t)start[s] * p->P,
-                             p->sens.p + (size_t)s * p->yf_elems, eps, p->osem_Dp, osem_astart_p(p, start[s]),
-                             p->osem_Db, osem_astart_b(p, start[s]));
-        }
+    // work buffers of the largest subset (allocated here, outside any capture)
+    p->u.ensure((size_t)maxsub * p->ustride, true, p->st);
+    p->part.ensure((size_t)p->osem_Dp * maxsub * p->P, false);
+    p->ratio.ensure((size_t)maxsub * p->P, false);
+    p->osem_start = start;
+    p->osem_cur_mu = mu_yf;
+    p->osem_cur_eps = eps;
+}
+
+// Device work of subset s (views start[s] .. start[s+1]-1 in subset order).
+void osem_subset(et_plan *p, int s)
+{
+    const std::vector<int> &start = p->osem_start;
+    const int nv = start[s + 1] - start[s];
+    const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
+    em_subset_update(p, r, nv, p->x_yf.p, p->osem_cur_mu, p->osem_meas.p + (size_t)start[s] * p->P,
+                     p->sens.p + (size_t)s * p->yf_elems, p->osem_cur_eps, p->osem_Dp, osem_astart_p(p, start[s]),
+                     p->osem_Db, osem_astart_b(p, start[s]));
+}
+
+void osem_finish(et_plan *p, float *x)
+{
     ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
     check_launch(et_launch_unpack_yf(p->x_yf.p, x, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
+}
+
+void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, int iters, float eps)
+{
+    if (S <= 0 || S > p->V) throw EtError{niftyrec_error_parameters};
+    if (!p->fast) {
+        if (S != 1) throw EtError{niftyrec_error_parameters};
+        for (int it = 0; it < iters; it++) em_step_general(p, x, meas, mu, eps);
+        return;
+    }
+    osem_prepare(p, x, meas, mu, S, eps);
+    for (int it = 0; it < iters; it++)
+        for (int s = 0; s < S; s++) osem_subset(p, s);
+    osem_finish(p, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1147,6 +1204,215 @@ int em_step_host(float *activity, int size_x, int size_y, const float *sino, int
 }
 
 thread_local char t_msg[2048];
+
+}  // namespace
+
+// ===========================================================================
+// Batches of plans: one OSEM iteration over many problems as a short list of
+// grouped launches (problems with the same PSF radius, block width and
+// attenuation presence share a launch), captured once into a hipGraph.
+// ===========================================================================
+
+struct et_batch {
+    int device = 0;
+    std::vector<et_plan *> plans;
+    hipStream_t own = nullptr;        // used when the caller passes the null stream
+    DevBuf tables;                    // argument records and block tables of every launch
+    struct Launch {
+        int kind;                     // 0 projection, 1 ratio, 2 rotated-frame backprojection, 3 pull + update
+        int R, nthr, has_mu;
+        size_t args_off, tab_off;     // byte offsets into `tables`
+        int n, blocks;
+    };
+    std::vector<Launch> launches;     // one OSEM iteration, in order
+    std::vector<const void *> key;    // buffers and chunkings the launch list refers to
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_stream = nullptr;
+    }
+    ~et_batch()
+    {
+        drop_graph();
+        if (own) (void)hipStreamDestroy(own);
+    }
+};
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct TableBytes {
+    std::vector<char> bytes;
+    template <class T>
+    size_t put(const std::vector<T> &v)
+    {
+        const size_t off = align256(bytes.size());
+        bytes.resize(off + v.size() * sizeof(T));
+        std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+// Block table of a grouped launch: {first block, gx, gy, 0} per problem, each
+// range padded to a multiple of 8 blocks.
+struct BlockTable {
+    std::vector<int> t;
+    int total = 0;
+    void add(int gx, int gy)
+    {
+        t.insert(t.end(), {total, gx, gy, 0});
+        total += ((gx * gy + 7) / 8) * 8;
+    }
+};
+
+std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
+{
+    std::vector<const void *> k;
+    k.push_back(reinterpret_cast<const void *>((uintptr_t)S));
+    uint32_t e;
+    std::memcpy(&e, &eps, 4);
+    k.push_back(reinterpret_cast<const void *>((uintptr_t)e));
+    for (const et_plan *p : b->plans) {
+        k.insert(k.end(), {p->x_yf.p, p->osem_meas.p, p->part.p, p->ratio.p, p->u.p, p->sens.p, p->osem_recs.p,
+                           p->osem_astart.p, p->osem_astart_b.p, p->osem_cur_mu, p->d_sep.p});
+        k.push_back(reinterpret_cast<const void *>((uintptr_t)(p->osem_Dp * 1024 + p->osem_Db)));
+    }
+    return k;
+}
+
+// Launch list of one OSEM iteration: per subset, grouped projections, the
+// ratios, grouped rotated-frame backprojections, then the pulls with the
+// multiplicative update.  The same argument records as em_subset_update.
+void batch_build(et_batch *b, int S, hipStream_t st)
+{
+    std::map<std::tuple<int, int, int>, std::vector<int>> groups;
+    for (size_t i = 0; i < b->plans.size(); i++) {
+        const et_plan *p = b->plans[i];
+        groups[std::make_tuple(p->R, nthreads_for(p->nx), p->osem_cur_mu ? 1 : 0)].push_back((int)i);
+    }
+    TableBytes tb;
+    std::vector<et_batch::Launch> L;
+    for (int s = 0; s < S; s++) {
+        for (int mode : {ET_MODE_PROJECT, ET_MODE_BPU}) {
+            if (mode == ET_MODE_BPU) {
+                std::vector<EtRatioArgs> ra;
+                BlockTable bt;
+                for (et_plan *p : b->plans) {
+                    const int v0 = p->osem_start[s], nv = p->osem_start[s + 1] - v0;
+                    const long long VP = (long long)nv * (long long)p->P;
+                    ra.push_back({p->osem_meas.p + (size_t)v0 * p->P, p->part.p, p->osem_Dp, VP, p->osem_cur_eps,
+                                  p->ratio.p});
+                    bt.add((int)((VP + 255) / 256), 1);
+                }
+                L.push_back({1, 0, 256, 0, tb.put(ra), tb.put(bt.t), (int)ra.size(), bt.total});
+            }
+            for (auto &g : groups) {
+                std::vector<EtSweepArgs> args;
+                BlockTable bt;
+                for (int i : g.second) {
+                    et_plan *p = b->plans[i];
+                    const int v0 = p->osem_start[s], nv = p->osem_start[s + 1] - v0;
+                    const float *r = p->osem_recs.p + (size_t)v0 * ET_VIEW_REC;
+                    const int ts = et_sweep_slab_rows(mode, p->R);
+                    if (mode == ET_MODE_PROJECT) {
+                        args.push_back(project_args(p, r, nv, p->x_yf.p, p->osem_cur_mu, p->part.p, 0, p->osem_Dp,
+                                                    osem_astart_p(p, v0)));
+                        bt.add((p->ny + ts - 1) / ts, nv * p->osem_Dp);
+                    } else {
+                        args.push_back(bpu_args(p, r, nv, p->osem_cur_mu, ET_SRC_SINO, p->ratio.p, nullptr, nullptr, 0,
+                                                0.f, p->osem_Db, osem_astart_b(p, v0), false, p->u.p));
+                        bt.add((p->ny + ts - 1) / ts, nv * p->osem_Db);
+                    }
+                }
+                L.push_back({mode == ET_MODE_PROJECT ? 0 : 2, std::get<0>(g.first), std::get<1>(g.first),
+                             std::get<2>(g.first), tb.put(args), tb.put(bt.t), (int)args.size(), bt.total});
+            }
+        }
+        std::vector<EtPullArgs> pa;
+        BlockTable bt;
+        for (et_plan *p : b->plans) {
+            const int v0 = p->osem_start[s], nv = p->osem_start[s + 1] - v0;
+            const float *r = p->osem_recs.p + (size_t)v0 * ET_VIEW_REC;
+            pa.push_back(pull_args(p, r, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, p->x_yf.p,
+                                   p->sens.p + (size_t)s * p->yf_elems));
+            bt.add((int)(((size_t)p->nx * p->nz + 255) / 256), (p->ny + ET_T - 1) / ET_T);
+        }
+        L.push_back({3, 0, 256, 0, tb.put(pa), tb.put(bt.t), (int)pa.size(), bt.total});
+    }
+    b->drop_graph();
+    b->tables.ensure((align256(tb.bytes.size()) + 3) / 4, false);
+    check_hip(hipMemcpyAsync(b->tables.p, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice, st),
+              niftyrec_error_transfergpu);
+    check_hip(hipStreamSynchronize(st), niftyrec_error_transfergpu);
+    b->launches = L;
+}
+
+void batch_issue(const et_batch *b, hipStream_t st)
+{
+    const char *base = reinterpret_cast<const char *>(b->tables.p);
+    for (const auto &l : b->launches) {
+        const void *args = base + l.args_off, *tab = base + l.tab_off;
+        const et_stream_t s = (et_stream_t)st;
+        switch (l.kind) {
+        case 0:
+        case 2:
+            check_launch(et_launch_sweep_group(l.kind == 0 ? ET_MODE_PROJECT : ET_MODE_BPU, l.R, l.has_mu, l.nthr,
+                                               static_cast<const EtSweepArgs *>(args), tab, l.n, l.blocks, s));
+            break;
+        case 1:
+            check_launch(et_launch_ratio_group(static_cast<const EtRatioArgs *>(args), tab, l.n, l.blocks, s));
+            break;
+        default:
+            check_launch(et_launch_pull_group(ET_EPI_OSEM, static_cast<const EtPullArgs *>(args), tab, l.n, l.blocks,
+                                              s));
+        }
+    }
+}
+
+void op_batch_osem(et_batch *b, float *const *act, const float *const *sino, const float *const *mu, int S,
+                   int iters, float eps, bool use_graph, hipStream_t st)
+{
+    for (size_t i = 0; i < b->plans.size(); i++) {
+        et_plan *p = b->plans[i];
+        p->st = st;
+        osem_prepare(p, act[i], sino[i], mu ? mu[i] : nullptr, S, eps);
+    }
+    const std::vector<const void *> key = batch_key(b, S, eps);
+    if (key != b->key || b->launches.empty()) {
+        batch_build(b, S, st);
+        b->key = key;
+    }
+    if (use_graph && (!b->exec || b->graph_stream != st)) {
+        b->drop_graph();
+        check_hip(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), niftyrec_error_kernel);
+        try {
+            batch_issue(b, st);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(st, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check_hip(hipStreamEndCapture(st, &b->graph), niftyrec_error_kernel);
+        check_hip(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), niftyrec_error_kernel);
+        b->graph_stream = st;
+    }
+    for (int it = 0; it < iters; it++) {
+        if (use_graph)
+            check_hip(hipGraphLaunch(b->exec, st), niftyrec_error_kernel);
+        else
+            batch_issue(b, st);
+    }
+    for (size_t i = 0; i < b->plans.size(); i++) osem_finish(b->plans[i], act[i]);
+}
 
 }  // namespace
 
@@ -1652,6 +1918,54 @@ int et_plan_profile_read(et_plan *plan, int category, int *launches, double *tot
     if (total_ms) *total_ms = plan->prof_ms[category];
     if (bytes) *bytes = plan->prof_bytes[category];
     return niftyrec_success;
+}
+
+int et_batch_create(et_batch **batch, et_plan *const *plans, int n_plans)
+{
+    if (!batch || !plans || n_plans <= 0) return niftyrec_error_parameters;
+    *batch = nullptr;
+    for (int i = 0; i < n_plans; i++)
+        if (!plans[i] || !plans[i]->fast || plans[i]->device != plans[0]->device) return niftyrec_error_parameters;
+    if (hipSetDevice(plans[0]->device) != hipSuccess) return niftyrec_error_nogpuinstalled;
+    return guarded([&]() -> int {
+        et_batch *b = new et_batch();
+        b->device = plans[0]->device;
+        b->plans.assign(plans, plans + n_plans);
+        if (hipStreamCreateWithFlags(&b->own, hipStreamNonBlocking) != hipSuccess) {
+            delete b;
+            return niftyrec_error_kernel;
+        }
+        *batch = b;
+        return niftyrec_success;
+    });
+}
+
+int et_batch_destroy(et_batch *batch)
+{
+    if (!batch) return niftyrec_error_parameters;
+    (void)hipSetDevice(batch->device);
+    delete batch;
+    return niftyrec_success;
+}
+
+int et_batch_osem(et_batch *batch, float *const *act, const float *const *sino, const float *const *mu, int subsets,
+                  int iterations, float epsilon, int use_graph, void *stream)
+{
+    if (!batch || !act || !sino || iterations < 0) return niftyrec_error_parameters;
+    for (size_t i = 0; i < batch->plans.size(); i++)
+        if (!act[i] || !sino[i] || subsets <= 0 || subsets > batch->plans[i]->V) return niftyrec_error_parameters;
+    if (hipSetDevice(batch->device) != hipSuccess) return niftyrec_error_nogpuinstalled;
+    return guarded([&]() -> int {
+        hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+        if (!st) {
+            // the null stream cannot be captured: order after it, run on the batch's own stream, finish
+            check_hip(hipStreamSynchronize(nullptr), niftyrec_error_kernel);
+            st = batch->own;
+        }
+        op_batch_osem(batch, act, sino, mu, subsets, iterations, epsilon, use_graph != 0, st);
+        if (st == batch->own) check_hip(hipStreamSynchronize(st), niftyrec_error_kernel);
+        return niftyrec_success;
+    });
 }
 
 }  // extern "C"

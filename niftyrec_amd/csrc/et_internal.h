@@ -93,6 +93,16 @@ struct EtPullArgs {
     const float *sens;       // ET_EPI_OSEM: clamped sensitivity (YF)
 };
 
+// MLEM ratio of one problem from its depth-chunk partial projections.
+struct EtRatioArgs {
+    const float *meas;       // [V][ny][nx]
+    const float *part;       // [D][V][ny][nx]
+    int D;
+    long long VP;            // V * ny * nx
+    float eps;
+    float *out;
+};
+
 // ---- launchers (et_kernels.hip) -------------------------------------------------
 // All return hipSuccess or the launch error; all are asynchronous on `stream`.
 typedef struct ihipStream_t *et_stream_t;
@@ -131,6 +141,18 @@ int et_launch_ratio_parts(const float *meas, const float *part, int D, size_t VP
                           et_stream_t stream);
 int et_launch_ratio(const float *meas, const float *proj, float *out, size_t n, float eps,
                     et_stream_t stream);
+// Grouped launches over several problems.  args: device array of n argument
+// records; tab: device array of n int4 {first block, gx, gy, 0} with first
+// blocks ascending multiples of 8 (each problem's gx*gy blocks, padded to a
+// multiple of 8); blocks: total grid.  Sweeps: project or BPU mode, every
+// problem with PSF radius R, the same mu presence and nthr = roundup(nx, 64)
+// threads; gx = ceil(ny / et_sweep_slab_rows(mode, R)), gy = V*D.  Pull:
+// gx = ceil(nx*nz / 256), gy = ceil(ny / ET_T).  Ratio: gx = ceil(VP/256), gy = 1.
+int et_launch_sweep_group(int mode, int R, int has_mu, int nthr, const EtSweepArgs *args, const void *tab, int n,
+                          int blocks, et_stream_t stream);
+int et_launch_pull_group(int epilogue, const EtPullArgs *args, const void *tab, int n, int blocks,
+                         et_stream_t stream);
+int et_launch_ratio_group(const EtRatioArgs *args, const void *tab, int n, int blocks, et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).
 int et_sweep_slab_rows(int mode, int R);
 // Sweep kernel variants compiled in (for diagnostics).

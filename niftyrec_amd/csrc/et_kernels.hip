@@ -17,6 +17,7 @@
 //
 // No MFMA: nothing here is a dense contraction.  Kernels never allocate.
 #include <hip/hip_runtime.h>
+#include <cstring>
 
 #include "et_internal.h"
 
@@ -367,28 +368,71 @@ struct SweepCtx {
     FastView fv;
 };
 
-// XCD-aware block order.  The dispatcher deals workgroups to the 8 XCDs
-// round-robin by linear id (x fastest), and each XCD has its own L2.  For a
-// grid of S fast x N slow blocks with N a multiple of 8, block i is given
-// fast index (i/8) % S and slow index ((i/8) / S) * 8 + i % 8: each XCD then
-// runs all S fast blocks of one slow index back to back, so the cache lines
-// they share (neighbouring axial slabs, halo rows) are fetched once per XCD.
+// Logical block of a launch: linear id (the dispatcher deals linear ids to the
+// 8 XCDs round-robin), coordinates and grid.  A single-problem launch uses its
+// own blockIdx/gridDim; a grouped launch maps its blocks onto the problems'
+// grids (each problem's block range starts at a multiple of 8, so lin & 7 is
+// the hardware XCD in both cases).
+struct Blk {
+    int lin, x, y, gx, gy;
+};
+
+__device__ __forceinline__ Blk blk_here()
+{
+    Blk b;
+    b.x = blockIdx.x;
+    b.y = blockIdx.y;
+    b.gx = gridDim.x;
+    b.gy = gridDim.y;
+    b.lin = b.x + b.gx * b.y;
+    return b;
+}
+
+// Grouped launches: entry j of `tab` = {first block, gx, gy, 0}, first blocks
+// ascending and multiples of 8.  Finds the problem of this block (binary
+// search; uniform per block) and its logical block; false for the padding
+// blocks at the end of a problem's range.
+__device__ __forceinline__ bool group_block(const int4 *__restrict__ tab, int n, Blk &b, int &j)
+{
+    const int L = blockIdx.x;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].x <= L) lo = mid;
+        else hi = mid - 1;
+    }
+    j = __builtin_amdgcn_readfirstlane(lo);   // uniform: keeps the argument record in SGPRs
+    const int4 t = tab[j];
+    const int local = L - t.x;
+    if (local >= t.y * t.z) return false;
+    b.lin = local;
+    b.gx = t.y;
+    b.gy = t.z;
+    b.x = local % t.y;
+    b.y = local / t.y;
+    return true;
+}
+
+// XCD-aware block order.  Each XCD has its own L2.  For a grid of S fast x N
+// slow blocks with N a multiple of 8, block lin is given fast index
+// (lin/8) % S and slow index ((lin/8) / S) * 8 + lin % 8: each XCD then runs
+// all S fast blocks of one slow index back to back, so the cache lines they
+// share (neighbouring axial slabs, halo rows) are fetched once per XCD.
 // Otherwise the launch order (fast0, slow0) is kept.  Returns (fast, slow).
-__device__ __forceinline__ int2 xcd_order(int S, int N, int fast0, int slow0)
+__device__ __forceinline__ int2 xcd_order(const Blk &b, int S, int N, int fast0, int slow0)
 {
     if ((N & 7) != 0) return make_int2(fast0, slow0);
-    const int i = blockIdx.x + gridDim.x * blockIdx.y;
-    const int seq = i >> 3;
-    return make_int2(seq % S, (seq / S) * 8 + (i & 7));
+    const int seq = b.lin >> 3;
+    return make_int2(seq % S, (seq / S) * 8 + (b.lin & 7));
 }
 
 template <int TS, int R>
-__device__ __forceinline__ SweepCtx sweep_ctx(const EtSweepArgs &a, bool chunked)
+__device__ __forceinline__ SweepCtx sweep_ctx(const EtSweepArgs &a, bool chunked, const Blk &b)
 {
     SweepCtx c;
     c.tx = threadIdx.x;
     c.nthr = blockDim.x;
-    const int2 fs = xcd_order(gridDim.x, gridDim.y, blockIdx.x, blockIdx.y);
+    const int2 fs = xcd_order(b, b.gx, b.gy, b.x, b.y);
     const int vd = fs.y;
     c.v = chunked ? vd / a.D : vd;
     c.d = chunked ? vd - c.v * a.D : 0;
@@ -422,13 +466,13 @@ __device__ __forceinline__ void zero_lds(float *lds, int n, int tx, int nthr)
 // PROJECT: partial projection of chunk d written to part[d][v][y][x]
 // (the sinogram itself when D == 1).  Reference emission-lib/_et.cpp:218-281.
 template <int TS, int R, bool HAS_MU, bool COOP>
-__global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
+__device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
 {
     constexpr int K = 2 * R + 1;
     constexpr int R4 = (R + 3) & ~3;
     constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
     extern __shared__ float lds[];
-    const SweepCtx c = sweep_ctx<TS, R>(a, true);
+    const SweepCtx c = sweep_ctx<TS, R>(a, true, b);
     zero_lds<R>(lds, c.bufsz, c.tx, c.nthr);
     // One row-pass buffer.  Each plane starts with a block barrier (the row
     // pass of the previous plane has been read by every wave); until its own
@@ -519,13 +563,13 @@ __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
 // BPU: u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ) for the slab rows,
 // reference emission-lib/_et.cpp:441-471 (line backprojection, then PSF).
 template <int TS, int R, bool HAS_MU, bool COOP>
-__global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
+__device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
 {
     constexpr int K = 2 * R + 1;
     constexpr int R4 = (R + 3) & ~3;
     constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
     extern __shared__ float lds[];
-    const SweepCtx c = sweep_ctx<TS, R>(a, true);
+    const SweepCtx c = sweep_ctx<TS, R>(a, true, b);
     zero_lds<R>(lds, c.bufsz, c.tx, c.nthr);
     // One row-pass buffer with cooperative staging in the wave's own slice of
     // it, as in k_project.
@@ -587,6 +631,43 @@ __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
 
 // Reverse sweeps for the attenuation gradients (one block per view and slab,
 // all of z').  A(z') = A_tot - sum_{k>z'} mu'(k), A_tot precomputed per ray.
+// Kernels of the projector and rotated-frame backprojection sweeps: one
+// problem (grid (slabs, views x chunks)), or a group of problems sharing the
+// template parameters and block size (1-D grid over the padded block ranges).
+template <int TS, int R, bool HAS_MU, bool COOP>
+__global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
+{
+    project_body<TS, R, HAS_MU, COOP>(a, blk_here());
+}
+
+template <int TS, int R, bool HAS_MU, bool COOP>
+__global__ void __launch_bounds__(512) k_project_group(const EtSweepArgs *__restrict__ args,
+                                                       const int4 *__restrict__ tab, int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = args[j];
+    project_body<TS, R, HAS_MU, COOP>(a, b);
+}
+
+template <int TS, int R, bool HAS_MU, bool COOP>
+__global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
+{
+    bpu_body<TS, R, HAS_MU, COOP>(a, blk_here());
+}
+
+template <int TS, int R, bool HAS_MU, bool COOP>
+__global__ void __launch_bounds__(512) k_bpu_group(const EtSweepArgs *__restrict__ args,
+                                                   const int4 *__restrict__ tab, int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = args[j];
+    bpu_body<TS, R, HAS_MU, COOP>(a, b);
+}
+
 // GRAD_REF: u = PSF( s * sum_{k>=z'} act'(k) e^{-A(k)} ), the reference kernel
 //   emission-lib_gpu/_et_attenuation_gradient_gpu_kernels.cu:15-42 followed by the
 //   PSF of emission-lib/_et.cpp:1829-1843.
@@ -597,7 +678,7 @@ __global__ void __launch_bounds__(512) k_grad_ref(EtSweepArgs a)
     constexpr int R4 = (R + 3) & ~3;
     constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
     extern __shared__ float lds[];
-    const SweepCtx c = sweep_ctx<TS, R>(a, false);
+    const SweepCtx c = sweep_ctx<TS, R>(a, false, blk_here());
     zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
     const int ys = c.y0 - R;
 
@@ -647,7 +728,7 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
     constexpr int R4 = (R + 3) & ~3;
     constexpr int NA = TS + 2 * R, LA = TS + 2 * R4, SA = R4 - R;
     extern __shared__ float lds[];
-    const SweepCtx c = sweep_ctx<TS, R>(a, false);
+    const SweepCtx c = sweep_ctx<TS, R>(a, false, blk_here());
     zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
 
     float sv[TS], At[TS], suf[TS], tail[TS];
@@ -694,7 +775,7 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
 template <int TS>
 __global__ void __launch_bounds__(512) k_musum(EtSweepArgs a)
 {
-    const SweepCtx c = sweep_ctx<TS, 0>(a, true);
+    const SweepCtx c = sweep_ctx<TS, 0>(a, true, blk_here());
     if (!c.xin) return;
     float acc[TS];
 #pragma unroll
@@ -831,12 +912,12 @@ __device__ __forceinline__ void yf_epilogue_coop(const EtPullArgs &a, const floa
 // every wave converged (columns past the grid sample a clamped column and
 // store nothing) because its sampler is wave-cooperative.
 template <bool ADJ, int EPI>
-__global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
+__device__ __forceinline__ void pull_body(const EtPullArgs &a, const Blk &b)
 {
     constexpr int SCR = ET_COOP_TAB + 64 * (T + 4);
     __shared__ float scr[ADJ ? 4 : 4 * SCR];
     // fast index: axial slab (neighbouring slabs share cache lines of u)
-    const int2 fs = xcd_order(gridDim.y, gridDim.x, blockIdx.y, blockIdx.x);
+    const int2 fs = xcd_order(b, b.gy, b.gx, b.y, b.x);
     const int cbase = fs.y * blockDim.x;
     const int c0 = cbase + threadIdx.x;
     const bool cin = c0 < a.nx * a.nz;
@@ -892,6 +973,25 @@ __global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
             }
         }
     }
+}
+
+// Pull kernels: one problem (grid (column blocks, axial slabs)) or a group of
+// problems (1-D grid over the padded block ranges).
+template <bool ADJ, int EPI>
+__global__ void __launch_bounds__(256) k_pull(EtPullArgs a)
+{
+    pull_body<ADJ, EPI>(a, blk_here());
+}
+
+template <bool ADJ, int EPI>
+__global__ void __launch_bounds__(256) k_pull_group(const EtPullArgs *__restrict__ args, const int4 *__restrict__ tab,
+                                                    int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtPullArgs a = args[j];
+    pull_body<ADJ, EPI>(a, b);
 }
 
 // ------------------------------------------------------------------------------
@@ -1236,6 +1336,23 @@ __global__ void __launch_bounds__(256) k_gather_planes(float *__restrict__ dst, 
 
 // OSEM ratio from partial projections, reference _et_array_interface.cpp:1098-1108:
 // p = max(sum_d part[d], 0); r = p < eps ? y / eps : y / p.
+// Grouped MLEM ratio over several problems (1-D grid; tab entry j =
+// {first block, blocks, 1, 0}).
+__global__ void __launch_bounds__(256) k_ratio_parts_group(const EtRatioArgs *__restrict__ args,
+                                                          const int4 *__restrict__ tab, int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtRatioArgs a = args[j];
+    const size_t i = (size_t)b.lin * blockDim.x + threadIdx.x;
+    if (i >= (size_t)a.VP) return;
+    float p = 0.f;
+    for (int d = 0; d < a.D; d++) p += a.part[(size_t)d * a.VP + i];
+    if (p < 0.f) p = 0.f;
+    a.out[i] = (p < a.eps) ? a.meas[i] / a.eps : a.meas[i] / p;
+}
+
 __global__ void __launch_bounds__(256) k_ratio_parts(const float *__restrict__ meas, const float *__restrict__ part,
                                                     int D, size_t VP, float eps, float *__restrict__ out)
 {
@@ -1302,10 +1419,18 @@ static void allow_lds(Kern k, size_t bytes)
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// Grouped launch description (device tables; see group_block).
+struct GroupLaunch {
+    const EtSweepArgs *args;
+    const int4 *tab;
+    int n, blocks, nthr;
+};
+
 template <int TS, int R>
-static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st)
+static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st,
+                           const GroupLaunch *grp = nullptr)
 {
-    const int nthr = ((a.nx + 63) / 64) * 64;
+    const int nthr = grp ? grp->nthr : ((a.nx + 63) / 64) * 64;
     const int slabs = (a.ny + TS - 1) / TS;
     const bool single = mode == ET_MODE_PROJECT || mode == ET_MODE_BPU;
     const bool coop = sweep_lds_bytes(R, TS, nthr, true, 1) <= lds_limit();
@@ -1313,10 +1438,16 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
     const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses, single ? 1 : 2);
     if (mode == ET_MODE_PROJECT) {
         dim3 g(slabs, a.V * a.D);
-#define ET_LP(mu, cp)                                                   \
-    {                                                                   \
-        allow_lds(k_project<TS, R, mu, cp>, lds);                       \
-        hipLaunchKernelGGL((k_project<TS, R, mu, cp>), g, dim3(nthr), lds, st, a); \
+#define ET_LP(mu, cp)                                                                                  \
+    {                                                                                                  \
+        if (grp) {                                                                                     \
+            allow_lds(k_project_group<TS, R, mu, cp>, lds);                                            \
+            hipLaunchKernelGGL((k_project_group<TS, R, mu, cp>), dim3(grp->blocks), dim3(nthr), lds, st, \
+                               grp->args, grp->tab, grp->n);                                           \
+        } else {                                                                                       \
+            allow_lds(k_project<TS, R, mu, cp>, lds);                                                  \
+            hipLaunchKernelGGL((k_project<TS, R, mu, cp>), g, dim3(nthr), lds, st, a);                 \
+        }                                                                                              \
     }
         if (has_mu) {
             if (coop) ET_LP(true, true) else ET_LP(true, false)
@@ -1326,10 +1457,16 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
 #undef ET_LP
     } else if (mode == ET_MODE_BPU) {
         dim3 g(slabs, a.V * a.D);
-#define ET_LB(mu, cp)                                               \
-    {                                                               \
-        allow_lds(k_bpu<TS, R, mu, cp>, lds);                       \
-        hipLaunchKernelGGL((k_bpu<TS, R, mu, cp>), g, dim3(nthr), lds, st, a); \
+#define ET_LB(mu, cp)                                                                                  \
+    {                                                                                                  \
+        if (grp) {                                                                                     \
+            allow_lds(k_bpu_group<TS, R, mu, cp>, lds);                                                \
+            hipLaunchKernelGGL((k_bpu_group<TS, R, mu, cp>), dim3(grp->blocks), dim3(nthr), lds, st,   \
+                               grp->args, grp->tab, grp->n);                                           \
+        } else {                                                                                       \
+            allow_lds(k_bpu<TS, R, mu, cp>, lds);                                                      \
+            hipLaunchKernelGGL((k_bpu<TS, R, mu, cp>), g, dim3(nthr), lds, st, a);                     \
+        }                                                                                              \
     }
         if (has_mu) {
             if (coop) ET_LB(true, true) else ET_LB(true, false)
@@ -1337,6 +1474,8 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
             ET_LB(false, false)
         }
 #undef ET_LB
+    } else if (grp) {
+        return (int)hipErrorInvalidValue;
     } else if (mode == ET_MODE_GRAD_REF) {
         hipLaunchKernelGGL((k_grad_ref<TS, R>), dim3(slabs, a.V), dim3(nthr), lds, st, a);
     } else if (mode == ET_MODE_GRAD_MU) {
@@ -1348,13 +1487,56 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
 }
 
 template <int R>
-static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st)
+static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st,
+                          const GroupLaunch *grp = nullptr)
 {
     switch (slab_rows(mode, R)) {
-    case 16: return launch_sweep_ts<16, R>(mode, has_mu, a, st);
-    case 8: return launch_sweep_ts<8, R>(mode, has_mu, a, st);
-    default: return launch_sweep_ts<4, R>(mode, has_mu, a, st);
+    case 16: return launch_sweep_ts<16, R>(mode, has_mu, a, st, grp);
+    case 8: return launch_sweep_ts<8, R>(mode, has_mu, a, st, grp);
+    default: return launch_sweep_ts<4, R>(mode, has_mu, a, st, grp);
     }
+}
+
+int et_launch_sweep_group(int mode, int R, int has_mu, int nthr, const EtSweepArgs *args, const void *tab, int n,
+                          int blocks, et_stream_t stream)
+{
+    if (mode != ET_MODE_PROJECT && mode != ET_MODE_BPU) return (int)hipErrorInvalidValue;
+    if (n <= 0 || blocks <= 0 || nthr <= 0 || nthr > 512 || (nthr & 63)) return (int)hipErrorInvalidValue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    GroupLaunch g{args, reinterpret_cast<const int4 *>(tab), n, blocks, nthr};
+    EtSweepArgs dummy;
+    memset(&dummy, 0, sizeof(dummy));
+    switch (R) {
+#define ET_R_CASE(r) \
+    case r: return launch_sweep_r<r>(mode, has_mu, dummy, st, &g);
+        ET_R_CASE(0) ET_R_CASE(1) ET_R_CASE(2) ET_R_CASE(3) ET_R_CASE(4) ET_R_CASE(5)
+        ET_R_CASE(6) ET_R_CASE(7) ET_R_CASE(8) ET_R_CASE(9) ET_R_CASE(10)
+#undef ET_R_CASE
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+int et_launch_pull_group(int epilogue, const EtPullArgs *args, const void *tab, int n, int blocks,
+                         et_stream_t stream)
+{
+    if (n <= 0 || blocks <= 0) return (int)hipErrorInvalidValue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int4 *t = reinterpret_cast<const int4 *>(tab);
+    if (epilogue == ET_EPI_OSEM)
+        hipLaunchKernelGGL((k_pull_group<false, ET_EPI_OSEM>), dim3(blocks), dim3(256), 0, st, args, t, n);
+    else if (epilogue == ET_EPI_YF_STORE)
+        hipLaunchKernelGGL((k_pull_group<false, ET_EPI_YF_STORE>), dim3(blocks), dim3(256), 0, st, args, t, n);
+    else
+        return (int)hipErrorInvalidValue;
+    return (int)hipGetLastError();
+}
+
+int et_launch_ratio_group(const EtRatioArgs *args, const void *tab, int n, int blocks, et_stream_t stream)
+{
+    if (n <= 0 || blocks <= 0) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ratio_parts_group, dim3(blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), args,
+                       reinterpret_cast<const int4 *>(tab), n);
+    return (int)hipGetLastError();
 }
 
 int et_launch_sweep(int mode, int R, int has_mu, int has_act, const EtSweepArgs &a, et_stream_t stream)

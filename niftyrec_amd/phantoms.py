@@ -101,3 +101,24 @@ def poisson_data(noiseless, counts, seed=0):
     rng = np.random.default_rng(seed)
     lam = noiseless.astype(np.float64) * (counts / max(float(noiseless.sum()), 1e-30))
     return rng.poisson(lam).astype(np.float32)
+
+
+# Volume shapes (nz, ny, nx) of the batched-reconstruction workload.
+BATCH_SHAPES = [(64, 64, 64), (96, 96, 96), (128, 128, 128), (160, 96, 160)]
+BATCH_PSF = {7: (0.5, 2.0), 11: (0.6, 3.0), 15: (0.7, 3.8)}
+
+
+def batch_problem(i):
+    """Problem i of the batched workload: seeded shape from BATCH_SHAPES, view
+    count U{60..180}, PSF size from {7, 11, 15} with a linear sigma ramp,
+    ellipsoid phantom with attenuation 0.15, counts proportional to the volume
+    (1e6 at 64^3)."""
+    rng = np.random.default_rng(1000 + i)
+    shape = BATCH_SHAPES[int(rng.integers(len(BATCH_SHAPES)))]
+    V = int(rng.integers(60, 181))
+    K = int(rng.choice([7, 11, 15]))
+    n_les = max(4, shape[2] // 16)
+    act, mu = phantom(shape, n_les, 1000 + i, lesion_axes=(2.0, max(4.0, shape[2] / 16.0)))
+    s0, s1 = BATCH_PSF[K]
+    return {"activity": act, "attenuation": mu, "psf": gaussian_psf(shape[0], K, s0, s1), "cameras": cameras(V),
+            "counts": 1e6 * act.size / 64 ** 3}
