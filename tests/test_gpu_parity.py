@@ -36,14 +36,21 @@ def _case(shape=(32, 24, 32), V=16, K=5, seed=0, mu_scale=0.1):
 
 
 def _report(name, got, ref32, ref64=None):
+    """relL2 <= 1e-5 against the reference; max_rel <= 1e-4 against the
+    reference, except where an FP64 result shows the reference itself further
+    than that from exact: then the HIP result's max_rel against FP64 may not
+    exceed the reference's own."""
     e32, m32 = rel_l2(got, ref32), max_rel(got, ref32)
     msg = "%s: relL2 %.2e max_rel %.2e vs reference" % (name, e32, m32)
     if ref64 is not None:
-        msg += "; relL2 %.2e vs FP64 oracle (reference itself: %.2e)" % (rel_l2(got, ref64), rel_l2(ref32, ref64))
+        msg += "; vs FP64 oracle: relL2 %.2e max_rel %.2e (reference itself: %.2e, %.2e)" % (
+            rel_l2(got, ref64), max_rel(got, ref64), rel_l2(ref32, ref64), max_rel(ref32, ref64))
     print(msg)
     assert e32 <= RTOL_L2, msg
-    assert m32 <= RTOL_MAX, msg
-    if ref64 is not None:
+    if ref64 is None:
+        assert m32 <= RTOL_MAX, msg
+    else:
+        assert m32 <= RTOL_MAX or max_rel(got, ref64) <= max_rel(ref32, ref64) + 1e-6, msg
         assert rel_l2(got, ref64) <= 2.0 * rel_l2(ref32, ref64) + 1e-7, msg
 
 
@@ -219,3 +226,28 @@ def test_spect64_full_size(lib):
     st, bp = refabi.backproject(lib, got, mu, cams, psf)
     assert st == 0
     _report("spect64 backproject", bp, orc.backproject(got, mu, cams, psf))
+
+
+def test_spect256_full_size_views():
+    """256^3 volume, PSF 21 (radius 10, the widest the sweep kernels take),
+    attenuation 0.15/0.05/0.30, on 4 of its 180 views (the oracle's cost),
+    through the device plan: projection, reference backprojection and exact
+    attenuation gradient against the oracle."""
+    torch = pytest.importorskip("torch")
+    from niftyrec_amd.api import Plan
+    w = phantoms.workload("spect256")
+    act, mu, psf = w["activity"], w["attenuation"], w["psf"]
+    cams = w["cameras"][[0, 37, 90, 151]]
+    plan = Plan(act.shape, cams, psf)
+    A, M = torch.from_numpy(act).cuda(), torch.from_numpy(mu).cuda()
+    p = plan.project(A, M).cpu().numpy()
+    _report("spect256 project (4 views)", p, orc.project(act, mu, cams, psf))
+    b = plan.backproject(torch.from_numpy(p).cuda(), M).cpu().numpy()
+    _report("spect256 backproject (4 views)", b, orc.backproject(p, mu, cams, psf),
+            orc.backproject(p, mu, cams, psf, dtype=np.float64))
+    s = np.random.default_rng(5).random(p.shape, dtype=np.float32)
+    g = plan.gradient_attenuation(torch.from_numpy(s).cuda(), A, M, exact=True, truncate=False).cpu().numpy()
+    ref = orc.gradient_mu(s, act, mu, cams, psf)
+    e = rel_l2(g, ref)
+    print("spect256 exact mu gradient (4 views): relL2 %.2e vs oracle" % e)
+    assert e <= RTOL_L2
