@@ -224,7 +224,7 @@ struct et_plan {
     int gy = ET_GY, yp = 0, xp = 0, zp = 0, yu = 0, slabs = 0;
     size_t yf_elems = 0, ustride = 0, P = 0, nvox = 0;
     // workspaces
-    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol;
+    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol, att;
     // OSEM caches
     DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart, osem_astart_b, osem_order;
     int osem_S = 0, osem_D = 0, osem_Dp = 0, osem_Db = 0;
@@ -456,9 +456,15 @@ void mu_prefix(et_plan *p, const float *recs, int nv, int D, const float *mu_yf,
     check_launch(et_launch_chunk_scan(p->csum.p, as, at, nv, D, (int)p->P, (et_stream_t)p->st));
 }
 
+// Floats of the attenuation-factor buffer of nv views ([nv][nz][yu][roundup(nx, 64)]).
+size_t att_elems(const et_plan *p, int nv)
+{
+    return (size_t)nv * p->nz * p->yu * (size_t)nthreads_for(p->nx);
+}
+
 // Arguments of a projection sweep of nv views into `part` ([D][nv][ny][nx]).
 EtSweepArgs project_args(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *part,
-                         int truncate, int D, const float *astart)
+                         int truncate, int D, const float *astart, float *att = nullptr)
 {
     EtSweepArgs a = base_args(p);
     a.views = recs;
@@ -470,21 +476,24 @@ EtSweepArgs project_args(et_plan *p, const float *recs, int nv, const float *act
     a.astart = astart;
     a.part = part;
     a.truncate = truncate;
+    a.att = mu_yf ? att : nullptr;
     return a;
 }
 
 // Fast-path projection of nv views (records recs) into sino [nv][ny][nx].
+// att: when set (with mu), the attenuation factors of the views are stored
+// there for a following rotated-frame backprojection of the same views.
 void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *sino,
-                  int truncate, int D, const float *astart, float *parts_out = nullptr)
+                  int truncate, int D, const float *astart, float *parts_out = nullptr, float *att = nullptr)
 {
     float *part = parts_out;
     if (!part) part = (D == 1) ? sino : p->part.ensure((size_t)D * nv * p->P, false);
     const EtSweepArgs a =
-        project_args(p, recs, nv, act_yf, mu_yf, part, (D == 1 && !parts_out) ? truncate : 0, D, astart);
+        project_args(p, recs, nv, act_yf, mu_yf, part, (D == 1 && !parts_out) ? truncate : 0, D, astart, att);
     {
         const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
         ProfScope ps(p, ET_PROF_PROJECT, bytes);
-        check_launch(et_launch_sweep(ET_MODE_PROJECT, p->R, mu_yf != nullptr, 1, a, (et_stream_t)p->st));
+        check_launch(et_launch_sweep(ET_MODE_PROJECT, p->R, mu_yf ? (a.att ? 2 : 1) : 0, 1, a, (et_stream_t)p->st));
     }
     if (D > 1 && !parts_out) {
         ProfScope ps(p, ET_PROF_OTHER, 0);
@@ -495,7 +504,7 @@ void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, co
 // Arguments of a rotated-frame backprojection sweep (see bpu_fast).
 EtSweepArgs bpu_args(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
                      const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
-                     float *u_out)
+                     float *u_out, float *att = nullptr)
 {
     EtSweepArgs a = base_args(p);
     a.views = recs;
@@ -511,19 +520,23 @@ EtSweepArgs bpu_args(et_plan *p, const float *recs, int nv, const float *mu_yf, 
     a.rD = rD;
     a.eps = eps;
     a.u = u_out;
+    a.att = mu_yf ? att : nullptr;
     if (p->K > 0) a.psf = flip ? p->d_sep_flip.p : p->d_sep.p;
     return a;
 }
 
 // u for nv views: PSF(s e^{-A}) (flip: correlation kernels).
+// att: attenuation factors of these views written by their projection (the
+// mu resampling is skipped), or null.
 void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
               const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
-              float *u_out)
+              float *u_out, float *att = nullptr)
 {
-    const EtSweepArgs a = bpu_args(p, recs, nv, mu_yf, src, sino, meas, rpart, rD, eps, D, astart, flip, u_out);
+    const EtSweepArgs a =
+        bpu_args(p, recs, nv, mu_yf, src, sino, meas, rpart, rD, eps, D, astart, flip, u_out, att);
     const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
     ProfScope ps(p, ET_PROF_BPU, bytes);
-    check_launch(et_launch_sweep(ET_MODE_BPU, p->R, mu_yf != nullptr, 0, a, (et_stream_t)p->st));
+    check_launch(et_launch_sweep(ET_MODE_BPU, p->R, mu_yf ? (a.att ? 2 : 1) : 0, 0, a, (et_stream_t)p->st));
 }
 
 EtPullArgs pull_args(et_plan *p, const float *recs, int nv, const float *u, bool adjoint, int epi, bool accumulate,
@@ -806,12 +819,14 @@ void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const 
     p->u.ensure((size_t)nv * p->ustride, true, p->st);
     float *parts = p->part.ensure((size_t)Dp * nv * p->P, false);
     float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
-    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, Dp, astart_p, parts);
+    // the projection leaves the attenuation factors of these views for the backprojection
+    float *att = mu_yf ? p->att.ensure(att_elems(p, nv), false) : nullptr;
+    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, Dp, astart_p, parts, att);
     {
         ProfScope ps(p, ET_PROF_OTHER, (Dp + 2.0) * 4.0 * nv * p->P);
         check_launch(et_launch_ratio_parts(meas, parts, Dp, (size_t)nv * p->P, eps, ratio, (et_stream_t)p->st));
     }
-    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, Db, astart_b, false, p->u.p);
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, Db, astart_b, false, p->u.p, att);
     pull_fast(p, recs, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, x_yf, sens_yf);
 }
 
@@ -890,11 +905,14 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
         }
         float *parts = p->part.ensure((size_t)D * nv * p->P, false);
         p->u.ensure((size_t)nv * p->ustride, true, p->st);
+        // as in em_subset_update: the projection stores the attenuation factors, the backprojection reads them
+        float *att = mu_yf ? p->att.ensure(att_elems(p, nv), true, p->st) : nullptr;
         auto run = [&]() {
             if (mode == ET_MODE_PROJECT)
-                project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts);
+                project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts, att);
             else
-                bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p);
+                bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
+                         att);
         };
         run();
         check_hip(hipEventRecord(e0, p->st), niftyrec_error_kernel);
@@ -989,6 +1007,7 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
     p->u.ensure((size_t)maxsub * p->ustride, true, p->st);
     p->part.ensure((size_t)p->osem_Dp * maxsub * p->P, false);
     p->ratio.ensure((size_t)maxsub * p->P, false);
+    if (mu_yf) p->att.ensure(att_elems(p, maxsub), false);
     p->osem_start = start;
     p->osem_cur_mu = mu_yf;
     p->osem_cur_eps = eps;
@@ -1282,7 +1301,7 @@ std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
     k.push_back(reinterpret_cast<const void *>((uintptr_t)e));
     for (const et_plan *p : b->plans) {
         k.insert(k.end(), {p->x_yf.p, p->osem_meas.p, p->part.p, p->ratio.p, p->u.p, p->sens.p, p->osem_recs.p,
-                           p->osem_astart.p, p->osem_astart_b.p, p->osem_cur_mu, p->d_sep.p});
+                           p->osem_astart.p, p->osem_astart_b.p, p->osem_cur_mu, p->d_sep.p, p->att.p});
         k.push_back(reinterpret_cast<const void *>((uintptr_t)(p->osem_Dp * 1024 + p->osem_Db)));
     }
     return k;
@@ -1324,16 +1343,18 @@ void batch_build(et_batch *b, int S, hipStream_t st)
                     const int ts = et_sweep_slab_rows(mode, p->R);
                     if (mode == ET_MODE_PROJECT) {
                         args.push_back(project_args(p, r, nv, p->x_yf.p, p->osem_cur_mu, p->part.p, 0, p->osem_Dp,
-                                                    osem_astart_p(p, v0)));
+                                                    osem_astart_p(p, v0), p->att.p));
                         bt.add((p->ny + ts - 1) / ts, nv * p->osem_Dp);
                     } else {
                         args.push_back(bpu_args(p, r, nv, p->osem_cur_mu, ET_SRC_SINO, p->ratio.p, nullptr, nullptr, 0,
-                                                0.f, p->osem_Db, osem_astart_b(p, v0), false, p->u.p));
+                                                0.f, p->osem_Db, osem_astart_b(p, v0), false, p->u.p, p->att.p));
                         bt.add((p->ny + ts - 1) / ts, nv * p->osem_Db);
                     }
                 }
-                L.push_back({mode == ET_MODE_PROJECT ? 0 : 2, std::get<0>(g.first), std::get<1>(g.first),
-                             std::get<2>(g.first), tb.put(args), tb.put(bt.t), (int)args.size(), bt.total});
+                // with mu: the projection stores the factors, the backprojection reads them (has_mu 2)
+                const int hm = std::get<2>(g.first) ? 2 : 0;
+                L.push_back({mode == ET_MODE_PROJECT ? 0 : 2, std::get<0>(g.first), std::get<1>(g.first), hm,
+                             tb.put(args), tb.put(bt.t), (int)args.size(), bt.total});
             }
         }
         std::vector<EtPullArgs> pa;

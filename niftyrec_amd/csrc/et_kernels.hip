@@ -465,7 +465,8 @@ __device__ __forceinline__ void zero_lds(float *lds, int n, int tx, int nthr)
 // grid: (slabs, V*D); block: nthr >= nx threads, thread tx owns column x' = tx.
 // PROJECT: partial projection of chunk d written to part[d][v][y][x]
 // (the sinogram itself when D == 1).  Reference emission-lib/_et.cpp:218-281.
-template <int TS, int R, bool HAS_MU, bool COOP>
+// ATT (with HAS_MU): also store the attenuation factors of every plane in a.att.
+template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
 __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
 {
     constexpr int K = 2 * R + 1;
@@ -539,8 +540,20 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             }
             row_pass<TS, R>(lds, c.tx, yc, kz, h);
         }
+        if constexpr (ATT) {
+            // factors of plane z for the backprojection, padded layout
+            // att[v][z][yu][nthr] (coalesced over x'; every lane stores)
+            float *ez = a.att + (((size_t)c.v * a.nz + z) * a.yu + c.y0) * c.nthr + c.tx;
 #pragma unroll
-        for (int i = 0; i < TS; i++) acc[i] = HAS_MU ? fmaf(h[i], exp_neg(A[i]), acc[i]) : acc[i] + h[i];
+            for (int i = 0; i < TS; i++) {
+                const float e = exp_neg(A[i]);
+                acc[i] = fmaf(h[i], e, acc[i]);
+                ez[i * c.nthr] = e;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < TS; i++) acc[i] = HAS_MU ? fmaf(h[i], exp_neg(A[i]), acc[i]) : acc[i] + h[i];
+        }
     }
     if (c.d == a.D - 1) {
 #pragma unroll
@@ -562,7 +575,9 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
 
 // BPU: u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ) for the slab rows,
 // reference emission-lib/_et.cpp:441-471 (line backprojection, then PSF).
-template <int TS, int R, bool HAS_MU, bool COOP>
+// ATT: the attenuation factors come from a.att (written by the projection of
+// the same views) instead of resampling mu.
+template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
 __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
 {
     constexpr int K = 2 * R + 1;
@@ -585,7 +600,8 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
         const int y = c.y0 - R + j;
         const bool ok = c.xin && y >= 0 && y < a.ny;
         sv[j] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
-        A[j] = (HAS_MU && ok && c.d > 0) ? a.astart[((size_t)c.v * a.D + c.d) * c.P + (size_t)y * a.nx + c.tx] : 0.f;
+        A[j] = (HAS_MU && !ATT && ok && c.d > 0) ? a.astart[((size_t)c.v * a.D + c.d) * c.P + (size_t)y * a.nx + c.tx]
+                                                 : 0.f;
     }
 
     for (int z = c.zb; z < c.ze; z++) {
@@ -593,7 +609,18 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         if (R > 0 && z > c.zb) __syncthreads();
         float yc[TS];
-        if constexpr (HAS_MU) {
+        if constexpr (HAS_MU && ATT) {
+            // rows outside the volume have s = 0: their factor is read from an
+            // in-range row and not used
+            const float *ez = a.att + ((size_t)c.v * a.nz + z) * a.yu * c.nthr + c.tx;
+            float rows[NA];
+#pragma unroll
+            for (int j = 0; j < NA; j++) {
+                const int yl = min(max(c.y0 - R + j, 0), a.ny - 1);
+                rows[j] = sv[j] * ez[yl * c.nthr];
+            }
+            column_pass<TS, R>(rows, kz + K, yc);
+        } else if constexpr (HAS_MU) {
             auto wfun = [&](int j, float m) {
                 A[j] += m;
                 return sv[j] * exp_neg(A[j]);
@@ -634,13 +661,13 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
 // Kernels of the projector and rotated-frame backprojection sweeps: one
 // problem (grid (slabs, views x chunks)), or a group of problems sharing the
 // template parameters and block size (1-D grid over the padded block ranges).
-template <int TS, int R, bool HAS_MU, bool COOP>
+template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
 __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
 {
-    project_body<TS, R, HAS_MU, COOP>(a, blk_here());
+    project_body<TS, R, HAS_MU, COOP, ATT>(a, blk_here());
 }
 
-template <int TS, int R, bool HAS_MU, bool COOP>
+template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
 __global__ void __launch_bounds__(512) k_project_group(const EtSweepArgs *__restrict__ args,
                                                        const int4 *__restrict__ tab, int n)
 {
@@ -648,16 +675,16 @@ __global__ void __launch_bounds__(512) k_project_group(const EtSweepArgs *__rest
     int j;
     if (!group_block(tab, n, b, j)) return;
     const EtSweepArgs a = args[j];
-    project_body<TS, R, HAS_MU, COOP>(a, b);
+    project_body<TS, R, HAS_MU, COOP, ATT>(a, b);
 }
 
-template <int TS, int R, bool HAS_MU, bool COOP>
+template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
 __global__ void __launch_bounds__(512) k_bpu(EtSweepArgs a)
 {
-    bpu_body<TS, R, HAS_MU, COOP>(a, blk_here());
+    bpu_body<TS, R, HAS_MU, COOP, ATT>(a, blk_here());
 }
 
-template <int TS, int R, bool HAS_MU, bool COOP>
+template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
 __global__ void __launch_bounds__(512) k_bpu_group(const EtSweepArgs *__restrict__ args,
                                                    const int4 *__restrict__ tab, int n)
 {
@@ -665,7 +692,7 @@ __global__ void __launch_bounds__(512) k_bpu_group(const EtSweepArgs *__restrict
     int j;
     if (!group_block(tab, n, b, j)) return;
     const EtSweepArgs a = args[j];
-    bpu_body<TS, R, HAS_MU, COOP>(a, b);
+    bpu_body<TS, R, HAS_MU, COOP, ATT>(a, b);
 }
 
 // GRAD_REF: u = PSF( s * sum_{k>=z'} act'(k) e^{-A(k)} ), the reference kernel
@@ -1434,44 +1461,48 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
     const int slabs = (a.ny + TS - 1) / TS;
     const bool single = mode == ET_MODE_PROJECT || mode == ET_MODE_BPU;
     const bool coop = sweep_lds_bytes(R, TS, nthr, true, 1) <= lds_limit();
-    const bool uses = mode == ET_MODE_PROJECT || (mode == ET_MODE_BPU && has_mu);
+    const bool uses = mode == ET_MODE_PROJECT || (mode == ET_MODE_BPU && has_mu == 1);
     const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses, single ? 1 : 2);
     if (mode == ET_MODE_PROJECT) {
         dim3 g(slabs, a.V * a.D);
-#define ET_LP(mu, cp)                                                                                  \
+#define ET_LP(mu, cp, at)                                                                              \
     {                                                                                                  \
         if (grp) {                                                                                     \
-            allow_lds(k_project_group<TS, R, mu, cp>, lds);                                            \
-            hipLaunchKernelGGL((k_project_group<TS, R, mu, cp>), dim3(grp->blocks), dim3(nthr), lds, st, \
-                               grp->args, grp->tab, grp->n);                                           \
+            allow_lds(k_project_group<TS, R, mu, cp, at>, lds);                                        \
+            hipLaunchKernelGGL((k_project_group<TS, R, mu, cp, at>), dim3(grp->blocks), dim3(nthr), lds, \
+                               st, grp->args, grp->tab, grp->n);                                       \
         } else {                                                                                       \
-            allow_lds(k_project<TS, R, mu, cp>, lds);                                                  \
-            hipLaunchKernelGGL((k_project<TS, R, mu, cp>), g, dim3(nthr), lds, st, a);                 \
+            allow_lds(k_project<TS, R, mu, cp, at>, lds);                                              \
+            hipLaunchKernelGGL((k_project<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);             \
         }                                                                                              \
     }
-        if (has_mu) {
-            if (coop) ET_LP(true, true) else ET_LP(true, false)
+        if (has_mu == 2) {
+            if (coop) ET_LP(true, true, true) else ET_LP(true, false, true)
+        } else if (has_mu) {
+            if (coop) ET_LP(true, true, false) else ET_LP(true, false, false)
         } else {
-            if (coop) ET_LP(false, true) else ET_LP(false, false)
+            if (coop) ET_LP(false, true, false) else ET_LP(false, false, false)
         }
 #undef ET_LP
     } else if (mode == ET_MODE_BPU) {
         dim3 g(slabs, a.V * a.D);
-#define ET_LB(mu, cp)                                                                                  \
+#define ET_LB(mu, cp, at)                                                                              \
     {                                                                                                  \
         if (grp) {                                                                                     \
-            allow_lds(k_bpu_group<TS, R, mu, cp>, lds);                                                \
-            hipLaunchKernelGGL((k_bpu_group<TS, R, mu, cp>), dim3(grp->blocks), dim3(nthr), lds, st,   \
+            allow_lds(k_bpu_group<TS, R, mu, cp, at>, lds);                                            \
+            hipLaunchKernelGGL((k_bpu_group<TS, R, mu, cp, at>), dim3(grp->blocks), dim3(nthr), lds, st, \
                                grp->args, grp->tab, grp->n);                                           \
         } else {                                                                                       \
-            allow_lds(k_bpu<TS, R, mu, cp>, lds);                                                      \
-            hipLaunchKernelGGL((k_bpu<TS, R, mu, cp>), g, dim3(nthr), lds, st, a);                     \
+            allow_lds(k_bpu<TS, R, mu, cp, at>, lds);                                                  \
+            hipLaunchKernelGGL((k_bpu<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);                 \
         }                                                                                              \
     }
-        if (has_mu) {
-            if (coop) ET_LB(true, true) else ET_LB(true, false)
+        if (has_mu == 2) {
+            ET_LB(true, false, true)
+        } else if (has_mu) {
+            if (coop) ET_LB(true, true, false) else ET_LB(true, false, false)
         } else {
-            ET_LB(false, false)
+            ET_LB(false, false, false)
         }
 #undef ET_LB
     } else if (grp) {
