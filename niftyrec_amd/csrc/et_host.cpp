@@ -226,7 +226,7 @@ struct et_plan {
     // workspaces
     DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol, att;
     // OSEM caches
-    DevBuf x_yf, sens, osem_recs, osem_meas, osem_astart, osem_astart_b, osem_order;
+    DevBuf x_yf, sens, osem_recs, osem_meas, osem_E, osem_order;
     int osem_S = 0, osem_D = 0, osem_Dp = 0, osem_Db = 0;
     bool osem_mu_yf = false;
     std::vector<int> osem_start;    // subset s = views osem_start[s] .. osem_start[s+1]-1 (subset order)
@@ -481,10 +481,11 @@ EtSweepArgs project_args(et_plan *p, const float *recs, int nv, const float *act
 }
 
 // Fast-path projection of nv views (records recs) into sino [nv][ny][nx].
-// att: when set (with mu), the attenuation factors of the views are stored
-// there for a following rotated-frame backprojection of the same views.
+// att (with mu): the attenuation factors of the views, stored there by this
+// projection (att_read false) or read from there instead of resampling mu.
 void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, const float *mu_yf, float *sino,
-                  int truncate, int D, const float *astart, float *parts_out = nullptr, float *att = nullptr)
+                  int truncate, int D, const float *astart, float *parts_out = nullptr, float *att = nullptr,
+                  bool att_read = false)
 {
     float *part = parts_out;
     if (!part) part = (D == 1) ? sino : p->part.ensure((size_t)D * nv * p->P, false);
@@ -493,7 +494,8 @@ void project_fast(et_plan *p, const float *recs, int nv, const float *act_yf, co
     {
         const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
         ProfScope ps(p, ET_PROF_PROJECT, bytes);
-        check_launch(et_launch_sweep(ET_MODE_PROJECT, p->R, mu_yf ? (a.att ? 2 : 1) : 0, 1, a, (et_stream_t)p->st));
+        const int hm = mu_yf ? (a.att ? (att_read ? 3 : 2) : 1) : 0;
+        check_launch(et_launch_sweep(ET_MODE_PROJECT, p->R, hm, 1, a, (et_stream_t)p->st));
     }
     if (D > 1 && !parts_out) {
         ProfScope ps(p, ET_PROF_OTHER, 0);
@@ -526,8 +528,7 @@ EtSweepArgs bpu_args(et_plan *p, const float *recs, int nv, const float *mu_yf, 
 }
 
 // u for nv views: PSF(s e^{-A}) (flip: correlation kernels).
-// att: attenuation factors of these views written by their projection (the
-// mu resampling is skipped), or null.
+// att: attenuation factors of these views (the mu resampling is skipped), or null.
 void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src, const float *sino,
               const float *meas, const float *rpart, int rD, float eps, int D, const float *astart, bool flip,
               float *u_out, float *att = nullptr)
@@ -536,7 +537,7 @@ void bpu_fast(et_plan *p, const float *recs, int nv, const float *mu_yf, int src
         bpu_args(p, recs, nv, mu_yf, src, sino, meas, rpart, rD, eps, D, astart, flip, u_out, att);
     const double bytes = (double)nv * ((mu_yf ? 8.0 : 4.0) * p->nvox + 4.0 * p->P);
     ProfScope ps(p, ET_PROF_BPU, bytes);
-    check_launch(et_launch_sweep(ET_MODE_BPU, p->R, mu_yf ? (a.att ? 2 : 1) : 0, 0, a, (et_stream_t)p->st));
+    check_launch(et_launch_sweep(ET_MODE_BPU, p->R, mu_yf ? (a.att ? 3 : 1) : 0, 0, a, (et_stream_t)p->st));
 }
 
 EtPullArgs pull_args(et_plan *p, const float *recs, int nv, const float *u, bool adjoint, int epi, bool accumulate,
@@ -814,29 +815,28 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
 // The projection runs in Dp depth chunks and the rotated-frame backprojection
 // in Db (each with its own attenuation prefix, null without mu or for one chunk).
 void em_subset_update(et_plan *p, const float *recs, int nv, float *x_yf, const float *mu_yf, const float *meas,
-                      const float *sens_yf, float eps, int Dp, const float *astart_p, int Db, const float *astart_b)
+                      const float *sens_yf, float eps, int Dp, int Db, float *att)
 {
     p->u.ensure((size_t)nv * p->ustride, true, p->st);
     float *parts = p->part.ensure((size_t)Dp * nv * p->P, false);
     float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
-    // the projection leaves the attenuation factors of these views for the backprojection
-    float *att = mu_yf ? p->att.ensure(att_elems(p, nv), false) : nullptr;
-    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, Dp, astart_p, parts, att);
+    project_fast(p, recs, nv, x_yf, mu_yf, nullptr, 1, Dp, nullptr, parts, att, true);
     {
         ProfScope ps(p, ET_PROF_OTHER, (Dp + 2.0) * 4.0 * nv * p->P);
         check_launch(et_launch_ratio_parts(meas, parts, Dp, (size_t)nv * p->P, eps, ratio, (et_stream_t)p->st));
     }
-    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, Db, astart_b, false, p->u.p, att);
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, Db, nullptr, false, p->u.p, att);
     pull_fast(p, recs, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, x_yf, sens_yf);
 }
 
 // Sensitivity of views [recs, nv): reference backprojection of ones, truncated,
 // then clamped below at eps (_et_array_interface.cpp:1112-1121), into sens_yf.
-void em_sensitivity(et_plan *p, const float *recs, int nv, const float *mu_yf, float eps, int D,
-                    const float *astart, float *sens_yf)
+// att: the views' attenuation factors (with mu).
+void em_sensitivity(et_plan *p, const float *recs, int nv, const float *mu_yf, float eps, int D, float *att,
+                    float *sens_yf)
 {
     p->u.ensure((size_t)nv * p->ustride, true, p->st);
-    bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, astart, false, p->u.p);
+    bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, nullptr, false, p->u.p, att);
     pull_fast(p, recs, nv, p->u.p, false, ET_EPI_YF_STORE, false, true, sens_yf);
     ProfScope ps(p, ET_PROF_OTHER, 0);
     check_launch(et_launch_yf_clamp_min(sens_yf, p->nx, p->ny, p->nz, p->yp, p->gy, eps, (et_stream_t)p->st));
@@ -862,19 +862,12 @@ void em_step_general(et_plan *p, float *x, const float *meas, const float *mu, f
     check_launch(et_launch_em_update_flat(x, b, s, p->nvox, (et_stream_t)p->st));
 }
 
-// Cached attenuation prefixes of the OSEM subsets (first view index v0), for
-// the projection and the rotated-frame backprojection chunkings.
-const float *osem_astart_p(et_plan *p, int v0)
+// Cached attenuation factors of the OSEM views from view v0 (subset order), or
+// null without attenuation.
+float *osem_att(et_plan *p, int v0)
 {
-    if (!p->osem_mu_yf || p->osem_Dp <= 1) return nullptr;
-    return p->osem_astart.p + (size_t)v0 * p->osem_Dp * p->P;
-}
-
-const float *osem_astart_b(et_plan *p, int v0)
-{
-    if (!p->osem_mu_yf || p->osem_Db <= 1) return nullptr;
-    if (p->osem_Db == p->osem_Dp) return osem_astart_p(p, v0);
-    return p->osem_astart_b.p + (size_t)v0 * p->osem_Db * p->P;
+    if (!p->osem_mu_yf) return nullptr;
+    return p->osem_E.p + (size_t)v0 * att_elems(p, 1);
 }
 
 // Depth chunking of the projection (mode ET_MODE_PROJECT) or rotated-frame
@@ -905,11 +898,11 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
         }
         float *parts = p->part.ensure((size_t)D * nv * p->P, false);
         p->u.ensure((size_t)nv * p->ustride, true, p->st);
-        // as in em_subset_update: the projection stores the attenuation factors, the backprojection reads them
+        // as in em_subset_update: both sweeps read the attenuation factors
         float *att = mu_yf ? p->att.ensure(att_elems(p, nv), true, p->st) : nullptr;
         auto run = [&]() {
             if (mode == ET_MODE_PROJECT)
-                project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts, att);
+                project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts, att, true);
             else
                 bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
                          att);
@@ -978,16 +971,24 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
         p->osem_Db = Db;
         p->osem_mu_yf = mu_yf != nullptr;
         p->sens.ensure((size_t)S * p->yf_elems, true, p->st);
-        if (mu_yf && Dp > 1) p->osem_astart.ensure((size_t)p->V * Dp * p->P, false);
-        if (mu_yf && Db > 1 && Db != Dp) p->osem_astart_b.ensure((size_t)p->V * Db * p->P, false);
+        // attenuation factors of every view, once per attenuation map: a
+        // projection sweep (of mu itself; its sums are discarded) stores them
+        if (mu_yf) p->osem_E.ensure(att_elems(p, p->V), false);
+        const int Dm = std::max(1, std::min(8, p->nz / 16));
         for (int s = 0; s < S; s++) {
             const int nv = start[s + 1] - start[s];
             const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
-            if (mu_yf && Dp > 1)
-                mu_prefix(p, r, nv, Dp, mu_yf, true, false, p->osem_astart.p + (size_t)start[s] * Dp * p->P);
-            if (mu_yf && Db > 1 && Db != Dp)
-                mu_prefix(p, r, nv, Db, mu_yf, true, false, p->osem_astart_b.p + (size_t)start[s] * Db * p->P);
-            em_sensitivity(p, r, nv, mu_yf, eps, Db, osem_astart_b(p, start[s]), p->sens.p + (size_t)s * p->yf_elems);
+            float *E = osem_att(p, start[s]);
+            if (mu_yf) {
+                const float *as = nullptr;
+                if (Dm > 1) {
+                    mu_prefix(p, r, nv, Dm, mu_yf, true, false);
+                    as = p->astart.p;
+                }
+                float *parts = p->part.ensure((size_t)Dm * nv * p->P, false);
+                project_fast(p, r, nv, mu_yf, mu_yf, nullptr, 0, Dm, as, parts, E, false);
+            }
+            em_sensitivity(p, r, nv, mu_yf, eps, Db, E, p->sens.p + (size_t)s * p->yf_elems);
         }
         check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
         p->osem_S = S;
@@ -1007,7 +1008,6 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
     p->u.ensure((size_t)maxsub * p->ustride, true, p->st);
     p->part.ensure((size_t)p->osem_Dp * maxsub * p->P, false);
     p->ratio.ensure((size_t)maxsub * p->P, false);
-    if (mu_yf) p->att.ensure(att_elems(p, maxsub), false);
     p->osem_start = start;
     p->osem_cur_mu = mu_yf;
     p->osem_cur_eps = eps;
@@ -1020,8 +1020,8 @@ void osem_subset(et_plan *p, int s)
     const int nv = start[s + 1] - start[s];
     const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
     em_subset_update(p, r, nv, p->x_yf.p, p->osem_cur_mu, p->osem_meas.p + (size_t)start[s] * p->P,
-                     p->sens.p + (size_t)s * p->yf_elems, p->osem_cur_eps, p->osem_Dp, osem_astart_p(p, start[s]),
-                     p->osem_Db, osem_astart_b(p, start[s]));
+                     p->sens.p + (size_t)s * p->yf_elems, p->osem_cur_eps, p->osem_Dp, p->osem_Db,
+                     osem_att(p, start[s]));
 }
 
 void osem_finish(et_plan *p, float *x)
@@ -1301,7 +1301,7 @@ std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
     k.push_back(reinterpret_cast<const void *>((uintptr_t)e));
     for (const et_plan *p : b->plans) {
         k.insert(k.end(), {p->x_yf.p, p->osem_meas.p, p->part.p, p->ratio.p, p->u.p, p->sens.p, p->osem_recs.p,
-                           p->osem_astart.p, p->osem_astart_b.p, p->osem_cur_mu, p->d_sep.p, p->att.p});
+                           p->osem_E.p, p->osem_cur_mu, p->d_sep.p});
         k.push_back(reinterpret_cast<const void *>((uintptr_t)(p->osem_Dp * 1024 + p->osem_Db)));
     }
     return k;
@@ -1343,16 +1343,16 @@ void batch_build(et_batch *b, int S, hipStream_t st)
                     const int ts = et_sweep_slab_rows(mode, p->R);
                     if (mode == ET_MODE_PROJECT) {
                         args.push_back(project_args(p, r, nv, p->x_yf.p, p->osem_cur_mu, p->part.p, 0, p->osem_Dp,
-                                                    osem_astart_p(p, v0), p->att.p));
+                                                    nullptr, osem_att(p, v0)));
                         bt.add((p->ny + ts - 1) / ts, nv * p->osem_Dp);
                     } else {
                         args.push_back(bpu_args(p, r, nv, p->osem_cur_mu, ET_SRC_SINO, p->ratio.p, nullptr, nullptr, 0,
-                                                0.f, p->osem_Db, osem_astart_b(p, v0), false, p->u.p, p->att.p));
+                                                0.f, p->osem_Db, nullptr, false, p->u.p, osem_att(p, v0)));
                         bt.add((p->ny + ts - 1) / ts, nv * p->osem_Db);
                     }
                 }
-                // with mu: the projection stores the factors, the backprojection reads them (has_mu 2)
-                const int hm = std::get<2>(g.first) ? 2 : 0;
+                // with mu: both sweeps read the cached attenuation factors (has_mu 3)
+                const int hm = std::get<2>(g.first) ? 3 : 0;
                 L.push_back({mode == ET_MODE_PROJECT ? 0 : 2, std::get<0>(g.first), std::get<1>(g.first), hm,
                              tb.put(args), tb.put(bt.t), (int)args.size(), bt.total});
             }

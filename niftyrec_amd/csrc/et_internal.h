@@ -69,8 +69,7 @@ struct EtSweepArgs {
     float eps;
     float *u;                // U layout outputs
     float *att;              // attenuation factors e^{-A(x',y,z')} of the views, [V][nz][yu][nthr] with
-                             // nthr = roundup(nx, 64) (the block width): PROJECT writes them, BPU reads them
-                             // instead of resampling mu (has_mu 2)
+                             // nthr = roundup(nx, 64) (the block width); see et_launch_sweep's has_mu
 };
 
 enum EtEpilogue {
@@ -110,9 +109,9 @@ struct EtRatioArgs {
 // All return hipSuccess or the launch error; all are asynchronous on `stream`.
 typedef struct ihipStream_t *et_stream_t;
 
-// has_mu: 0 no attenuation; 1 attenuation resampled from a.mu; 2 as 1 and, in
-// PROJECT, the attenuation factors stored in a.att, in BPU, read from a.att
-// instead of resampling mu.
+// has_mu: 0 no attenuation; 1 attenuation from the resampled a.mu; 2 (PROJECT)
+// as 1, and the attenuation factors stored in a.att; 3 (PROJECT, BPU) the
+// factors read from a.att instead of resampling mu.
 int et_launch_sweep(int mode, int R, int has_mu, int has_act, const EtSweepArgs &a,
                     et_stream_t stream);
 int et_launch_pull(const EtPullArgs &a, et_stream_t stream);

@@ -465,8 +465,10 @@ __device__ __forceinline__ void zero_lds(float *lds, int n, int tx, int nthr)
 // grid: (slabs, V*D); block: nthr >= nx threads, thread tx owns column x' = tx.
 // PROJECT: partial projection of chunk d written to part[d][v][y][x]
 // (the sinogram itself when D == 1).  Reference emission-lib/_et.cpp:218-281.
-// ATT (with HAS_MU): also store the attenuation factors of every plane in a.att.
-template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
+// AM (with HAS_MU): 0 attenuation from the resampled mu; 1 the same, and the
+// factors e^{-A} of every plane stored in a.att; 2 the factors read from a.att
+// (mu is not read).
+template <int TS, int R, bool HAS_MU, bool COOP, int AM>
 __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
 {
     constexpr int K = 2 * R + 1;
@@ -492,7 +494,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
         A[i] = 0.f;
         acc[i] = 0.f;
     }
-    if (HAS_MU && c.d > 0 && c.xin) {
+    if (HAS_MU && AM != 2 && c.d > 0 && c.xin) {
         const float *as = a.astart + ((size_t)c.v * a.D + c.d) * c.P;
 #pragma unroll
         for (int i = 0; i < TS; i++) {
@@ -515,7 +517,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
                                                [](int, float v) { return v; }, yc);
         }
-        if constexpr (HAS_MU) {
+        if constexpr (HAS_MU && AM != 2) {
             // attenuation up to and including plane z (used after the row pass)
             auto cum = [&](int i, float m) {
                 A[i] += m;
@@ -540,24 +542,36 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             }
             row_pass<TS, R>(lds, c.tx, yc, kz, h);
         }
-        if constexpr (ATT) {
-            // factors of plane z for the backprojection, padded layout
-            // att[v][z][yu][nthr] (coalesced over x'; every lane stores)
-            float *ez = a.att + (((size_t)c.v * a.nz + z) * a.yu + c.y0) * c.nthr + c.tx;
+        // factors of plane z in the padded layout att[v][z][yu][nthr]
+        // (coalesced over x'; every lane stores or loads)
+        float *ez = a.att + (((size_t)c.v * a.nz + z) * a.yu + c.y0) * c.nthr + c.tx;
+        if constexpr (HAS_MU && AM == 1) {
 #pragma unroll
             for (int i = 0; i < TS; i++) {
                 const float e = exp_neg(A[i]);
                 acc[i] = fmaf(h[i], e, acc[i]);
                 ez[i * c.nthr] = e;
             }
+        } else if constexpr (HAS_MU && AM == 2) {
+#pragma unroll
+            for (int i = 0; i < TS; i++) acc[i] = fmaf(h[i], ez[i * c.nthr], acc[i]);
         } else {
 #pragma unroll
             for (int i = 0; i < TS; i++) acc[i] = HAS_MU ? fmaf(h[i], exp_neg(A[i]), acc[i]) : acc[i] + h[i];
         }
     }
     if (c.d == a.D - 1) {
+        if constexpr (HAS_MU && AM == 2) {
+            // e^{-A} along the whole ray = the factor of the last plane
+            if (a.bg_line != 0.f) {
+                const float *el = a.att + (((size_t)c.v * a.nz + a.nz - 1) * a.yu + c.y0) * c.nthr + c.tx;
 #pragma unroll
-        for (int i = 0; i < TS; i++) acc[i] += HAS_MU ? a.bg_line * exp_neg(A[i]) : a.bg_line;
+                for (int i = 0; i < TS; i++) acc[i] += a.bg_line * el[i * c.nthr];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < TS; i++) acc[i] += HAS_MU ? a.bg_line * exp_neg(A[i]) : a.bg_line;
+        }
     }
     if (c.xin) {
         float *out = a.part + ((size_t)c.d * a.V + c.v) * c.P;
@@ -661,13 +675,13 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
 // Kernels of the projector and rotated-frame backprojection sweeps: one
 // problem (grid (slabs, views x chunks)), or a group of problems sharing the
 // template parameters and block size (1-D grid over the padded block ranges).
-template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
+template <int TS, int R, bool HAS_MU, bool COOP, int AM>
 __global__ void __launch_bounds__(512) k_project(EtSweepArgs a)
 {
-    project_body<TS, R, HAS_MU, COOP, ATT>(a, blk_here());
+    project_body<TS, R, HAS_MU, COOP, AM>(a, blk_here());
 }
 
-template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
+template <int TS, int R, bool HAS_MU, bool COOP, int AM>
 __global__ void __launch_bounds__(512) k_project_group(const EtSweepArgs *__restrict__ args,
                                                        const int4 *__restrict__ tab, int n)
 {
@@ -675,7 +689,7 @@ __global__ void __launch_bounds__(512) k_project_group(const EtSweepArgs *__rest
     int j;
     if (!group_block(tab, n, b, j)) return;
     const EtSweepArgs a = args[j];
-    project_body<TS, R, HAS_MU, COOP, ATT>(a, b);
+    project_body<TS, R, HAS_MU, COOP, AM>(a, b);
 }
 
 template <int TS, int R, bool HAS_MU, bool COOP, bool ATT>
@@ -1476,12 +1490,14 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
             hipLaunchKernelGGL((k_project<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);             \
         }                                                                                              \
     }
-        if (has_mu == 2) {
-            if (coop) ET_LP(true, true, true) else ET_LP(true, false, true)
+        if (has_mu == 3) {
+            if (coop) ET_LP(true, true, 2) else ET_LP(true, false, 2)
+        } else if (has_mu == 2) {
+            if (coop) ET_LP(true, true, 1) else ET_LP(true, false, 1)
         } else if (has_mu) {
-            if (coop) ET_LP(true, true, false) else ET_LP(true, false, false)
+            if (coop) ET_LP(true, true, 0) else ET_LP(true, false, 0)
         } else {
-            if (coop) ET_LP(false, true, false) else ET_LP(false, false, false)
+            if (coop) ET_LP(false, true, 0) else ET_LP(false, false, 0)
         }
 #undef ET_LP
     } else if (mode == ET_MODE_BPU) {
@@ -1497,7 +1513,7 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
             hipLaunchKernelGGL((k_bpu<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);                 \
         }                                                                                              \
     }
-        if (has_mu == 2) {
+        if (has_mu == 3) {
             ET_LB(true, false, true)
         } else if (has_mu) {
             if (coop) ET_LB(true, true, false) else ET_LB(true, false, false)
