@@ -891,21 +891,22 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
     int best = cand[0];
     float best_ms = 1e30f;
     for (int D : cand) {
-        const float *as = nullptr;
-        if (mu_yf) {
-            mu_prefix(p, recs, nv, D, mu_yf, true, false);
-            as = p->astart.p;
-        }
+        const float *as = nullptr;  // both sweeps read the factors: no attenuation prefix
         float *parts = p->part.ensure((size_t)D * nv * p->P, false);
         p->u.ensure((size_t)nv * p->ustride, true, p->st);
         // as in em_subset_update: both sweeps read the attenuation factors
         float *att = mu_yf ? p->att.ensure(att_elems(p, nv), true, p->st) : nullptr;
+        float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
         auto run = [&]() {
-            if (mode == ET_MODE_PROJECT)
+            if (mode == ET_MODE_PROJECT) {
+                // with the ratio kernel, whose reads grow with the number of partial sums
                 project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts, att, true);
-            else
+                check_launch(et_launch_ratio_parts(parts, parts, D, (size_t)nv * p->P, 1e-6f, ratio,
+                                                   (et_stream_t)p->st));
+            } else {
                 bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
                          att);
+            }
         };
         run();
         check_hip(hipEventRecord(e0, p->st), niftyrec_error_kernel);

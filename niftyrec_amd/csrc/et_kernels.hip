@@ -359,6 +359,34 @@ __device__ __forceinline__ void store_u(const EtSweepArgs &a, int v, int tx, int
     for (int q = 0; q < TS / 4; q++) d4[q] = make_float4(val[4 * q], val[4 * q + 1], val[4 * q + 2], val[4 * q + 3]);
 }
 
+// Coalesced form of store_u for the waves of a sweep block with a PSF: the
+// wave's TS-row runs go through `stage` (its own slice of the row-pass buffer,
+// row stride TS+4, idle until the next row-pass write) so that TS/4
+// consecutive lanes store the aligned float4 chunks of one run.  Every lane of
+// the wave calls it; columns x' >= nx store nothing.
+template <int TS>
+__device__ __forceinline__ void store_u_coop(const EtSweepArgs &a, int v, int xbase, int z, int y0, float *stage,
+                                             const float *val)
+{
+    constexpr int L = TS + 4, CHS = TS / 4, CPR = 64 / CHS;
+    const int lane = threadIdx.x & 63;
+    float4 *w4 = reinterpret_cast<float4 *>(stage + lane * L);
+#pragma unroll
+    for (int q = 0; q < CHS; q++) w4[q] = make_float4(val[4 * q], val[4 * q + 1], val[4 * q + 2], val[4 * q + 3]);
+    wave_sync();
+    const int k = lane % CHS, tl = lane / CHS;
+    float *uz = a.u + (size_t)v * a.ustride + (size_t)a.yu * a.xp * (z + 1) + y0 + 4 * k;
+#pragma unroll
+    for (int r = 0; r < CHS; r++) {
+        const int t = r * CPR + tl;
+        const int xq = xbase + t;
+        if (xq < a.nx)
+            *reinterpret_cast<float4 *>(uz + (size_t)a.yu * (xq + 1)) =
+                *reinterpret_cast<const float4 *>(stage + t * L + 4 * k);
+    }
+    wave_sync();
+}
+
 // Common geometry of a sweep block.
 struct SweepCtx {
     int tx, nthr, v, d, y0, zb, ze, dx, dz, bufsz;
@@ -618,10 +646,18 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
                                                  : 0.f;
     }
 
+    // with a PSF, plane z's output is stored during plane z+1, after its
+    // barrier, through the wave's idle row-buffer slice (coalesced)
+    float outp[TS];
+    float *own = lds + (64 * wv + R) * L;
+    const int xbase = c.tx & ~63;
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
-        if (R > 0 && z > c.zb) __syncthreads();
+        if (R > 0 && z > c.zb) {
+            __syncthreads();
+            store_u_coop<TS>(a, c.v, xbase, z - 1, c.y0, own, outp);
+        }
         float yc[TS];
         if constexpr (HAS_MU && ATT) {
             // rows outside the volume have s = 0: their factor is read from an
@@ -659,14 +695,17 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
                 yc[i] = t;
             }
         }
-        float out[TS];
         if constexpr (R == 0) {
-#pragma unroll
-            for (int i = 0; i < TS; i++) out[i] = yc[i];
+            if (c.xin) store_u<TS>(a, c.v, c.tx, z, c.y0, yc);
         } else {
-            row_pass<TS, R>(lds, c.tx, yc, kz, out);
+            row_pass<TS, R>(lds, c.tx, yc, kz, outp);
         }
-        if (c.xin) store_u<TS>(a, c.v, c.tx, z, c.y0, out);
+    }
+    if constexpr (R > 0) {
+        if (c.ze > c.zb) {
+            __syncthreads();
+            store_u_coop<TS>(a, c.v, xbase, c.ze - 1, c.y0, own, outp);
+        }
     }
 }
 
