@@ -201,8 +201,13 @@ int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, 
                  int iterations, float epsilon, void *stream);
 /* Tuning: depth chunks per ray (0 = automatic), views per backprojection batch (0 = automatic). */
 int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch);
-/* Kernel timing with HIP events on the plan's launches (category ids below). */
+/* Kernel timing with HIP events on the plan's launches (category ids below):
+ * et_plan_profile times every category (enable != 0) or none; _select times
+ * the categories whose bits are set in `categories` (bit ET_PROF_*), so that
+ * a caller can time one kernel with fewer event records in the stream.  Both
+ * reset the accumulated figures. */
 int et_plan_profile(et_plan *plan, int enable);
+int et_plan_profile_select(et_plan *plan, unsigned int categories);
 int et_plan_profile_read(et_plan *plan, int category, int *launches, double *total_ms,
                          double *bytes);
 #define ET_PROF_PROJECT 0

@@ -255,8 +255,16 @@ class Plan:
     def set_tuning(self, depth_chunks=0, view_batch=0):
         _lib.check(self._L.et_plan_set_tuning(self._h, int(depth_chunks), int(view_batch)), "et_plan_set_tuning")
 
-    def profile(self, enable=True):
-        _lib.check(self._L.et_plan_profile(self._h, int(bool(enable))), "et_plan_profile")
+    def profile(self, enable=True, categories=None):
+        """Time the plan's launches with HIP events: every category, or only the
+        named ones (names of _lib.PROF_NAMES); resets the figures."""
+        if categories is None:
+            _lib.check(self._L.et_plan_profile(self._h, int(bool(enable))), "et_plan_profile")
+            return
+        mask = 0
+        for name in categories:
+            mask |= 1 << _lib.PROF_NAMES.index(name)
+        _lib.check(self._L.et_plan_profile_select(self._h, mask if enable else 0), "et_plan_profile_select")
 
     def profile_read(self):
         """{category: (launches, total_ms, algorithmic_bytes)}."""

@@ -237,7 +237,8 @@ struct et_plan {
     // tuning
     int tune_D = 0, tune_vb = 0;
     // profiling
-    bool prof = false;
+    unsigned prof = 0;              // bit c set: launches of profiling category c are timed
+    std::vector<hipEvent_t> ev_pool;  // events reused by the profiling scopes
     struct Ev {
         int cat;
         double bytes;
@@ -254,6 +255,7 @@ struct et_plan {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
         }
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     }
 };
 
@@ -336,6 +338,19 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
 
 // ---- profiling --------------------------------------------------------------
 
+hipEvent_t pool_event(et_plan *p)
+{
+    if (!p->ev_pool.empty()) {
+        hipEvent_t e = p->ev_pool.back();
+        p->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    return hipEventCreate(&e) == hipSuccess ? e : nullptr;
+}
+
+// Times the launches issued during its lifetime when category `cat` is
+// selected (et_plan_profile_select); the events come from the plan's pool.
 struct ProfScope {
     et_plan *p;
     int cat;
@@ -343,16 +358,20 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(et_plan *p_, int cat_, double bytes_) : p(p_), cat(cat_), bytes(bytes_)
     {
-        if (!p->prof) return;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-            p->prof = false;
+        if (!((p->prof >> cat) & 1u)) return;
+        a = pool_event(p);
+        b = pool_event(p);
+        if (!a || !b) {
+            if (a) p->ev_pool.push_back(a);
+            if (b) p->ev_pool.push_back(b);
+            a = b = nullptr;
             return;
         }
         (void)hipEventRecord(a, p->st);
     }
     ~ProfScope()
     {
-        if (!p->prof || !a) return;
+        if (!a) return;
         (void)hipEventRecord(b, p->st);
         p->events.push_back({cat, bytes, a, b});
     }
@@ -369,8 +388,8 @@ void prof_collect(et_plan *p)
             p->prof_bytes[e.cat] += e.bytes;
             p->prof_n[e.cat] += 1;
         }
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
+        p->ev_pool.push_back(e.a);
+        p->ev_pool.push_back(e.b);
     }
     p->events.clear();
 }
@@ -885,8 +904,8 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
     hipEvent_t e0, e1;
     check_hip(hipEventCreate(&e0), niftyrec_error_kernel);
     check_hip(hipEventCreate(&e1), niftyrec_error_kernel);
-    const bool prof = p->prof;
-    p->prof = false;
+    const unsigned prof = p->prof;
+    p->prof = 0;
     const float *act = mu_yf ? mu_yf : p->act_yf.ensure(p->yf_elems, true, p->st);
     int best = cand[0];
     float best_ms = 1e30f;
@@ -1921,9 +1940,14 @@ int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch_)
 
 int et_plan_profile(et_plan *plan, int enable)
 {
+    return et_plan_profile_select(plan, enable ? (1u << ET_PROF_COUNT) - 1u : 0u);
+}
+
+int et_plan_profile_select(et_plan *plan, unsigned int categories)
+{
     if (!plan) return niftyrec_error_parameters;
     prof_collect(plan);
-    plan->prof = enable != 0;
+    plan->prof = categories & ((1u << ET_PROF_COUNT) - 1u);
     for (int i = 0; i < ET_PROF_COUNT; i++) {
         plan->prof_ms[i] = 0;
         plan->prof_bytes[i] = 0;
