@@ -102,6 +102,19 @@ int et_array_backproject(float *sino, int *sino_size, float *bkpr, int *bkpr_siz
                          float *attenuation, int *attenuation_size, float background,
                          float background_attenuation, int GPU, int truncate_negative_values);
 
+/* Partial projection: sinogram plus, for every camera, the partial line
+ * integrals (what each ray has collected beyond each plane), in the image
+ * frame when do_rotate_partial, else in the rotated frame; partialsum is
+ * [N, m, N, n_cameras].  Reference _et_array_interface.h:47,
+ * _et_array_interface.cpp:1561-1787 (CUDA path _et.cpp:2106-2440; the
+ * reference CPU path is a stub returning 1). */
+int et_array_project_partial(float *activity, int *activity_size, float *sinogram,
+                             int *sinogram_size, float *partialsum, int *partialsum_size,
+                             float *cameras, int *cameras_size, float *psf, int *psf_size,
+                             float *attenuation, int *attenuation_size, float background,
+                             float *background_image, float background_attenuation, int GPU,
+                             int truncate_negative_values, int do_rotate_partial);
+
 /* Reference attenuation gradient, emission-lib/_et_array_interface.h:48
  * (_et_array_interface.cpp:714-892; CUDA implementation emission-lib/_et.cpp:1679-1924). */
 int et_array_gradient_attenuation(float *sino, int *sino_size, float *activity,

@@ -37,6 +37,8 @@ SIGNATURES = {
     "et_array_backproject": (c_int, [fp, ip, fp, ip, fp, ip, fp, ip, fp, ip, c_float, c_float, c_int, c_int]),
     "et_array_gradient_attenuation": (c_int, [fp, ip, fp, ip, fp, ip, fp, ip, fp, ip, fp, ip, c_float, c_float,
                                               c_int, c_int]),
+    "et_array_project_partial": (c_int, [fp, ip, fp, ip, fp, ip, fp, ip, fp, ip, fp, ip, c_float, fp, c_float,
+                                         c_int, c_int, c_int]),
     "et_array_rotate": (c_int, [fp, ip, fp, fp, fp, c_float, c_int, c_int]),
     "et_array_affine": (c_int, [fp, ip, fp, ip, fp, ip, c_float, c_int]),
     "et_array_convolve": (c_int, [fp, ip, fp, ip, fp, ip, c_int]),

@@ -61,6 +61,27 @@ def project(activity, cameras, psf=None, attenuation=None, background=0.0, backg
     return sino
 
 
+def project_partial(activity, cameras, psf=None, attenuation=None, background=0.0, background_image=None,
+                    background_attenuation=0.0, truncate_negative_values=True, rotate_partial=True):
+    """Projection plus the partial line integrals of every view:
+    (sinogram (V, ny, nx), partial sums (V, nz, ny, nx)); the partial sums are
+    in the image frame when rotate_partial, else in each view's rotated frame."""
+    L = _lib.load()
+    act, mu, psf, bgi = _f32(activity), _f32(attenuation), _f32(psf), _f32(background_image)
+    ref = act if act is not None else mu
+    cams, cols, V = _cams(cameras)
+    nz, ny, nx = ref.shape
+    sino = np.zeros((V, ny, nx), dtype=np.float32)
+    part = np.zeros((V, nz, ny, nx), dtype=np.float32)
+    st = L.et_array_project_partial(_p(act), _size(act), _p(sino), (ctypes.c_int * 3)(nx, ny, V), _p(part),
+                                    (ctypes.c_int * 4)(nx, ny, nz, V), _p(cams), (ctypes.c_int * 2)(V, cols),
+                                    _p(psf), _size(psf), _p(mu), _size(mu), float(background), _p(bgi),
+                                    float(background_attenuation), 1, int(bool(truncate_negative_values)),
+                                    int(bool(rotate_partial)))
+    _lib.check(st, "project_partial")
+    return sino, part
+
+
 def backproject(sinogram, cameras, psf=None, attenuation=None, background=0.0, background_attenuation=0.0,
                 truncate_negative_values=True, nz=None):
     """Reference backprojection (inverse-rotation pull), volume (nz, ny, nx)."""

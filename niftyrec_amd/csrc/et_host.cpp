@@ -1707,6 +1707,117 @@ int et_array_affine(float *image, int *image_size, float *transformed_image, int
     });
 }
 
+// Partial projection, reference _et_array_interface.cpp:1561-1787 and the
+// CUDA path _et.cpp:2106-2440 (the reference CPU path is a stub).  Per view:
+// activity and attenuation resampled on the general path (pads background,
+// background_attenuation), activity blurred by the separated PSF, partial line
+// integrals (k_partial_line), then the partial sums pulled back to the image
+// frame (negated angles, ZYX order, pad background) when do_rotate_partial.
+// partialsum: [nx, ny, nz, n_cameras] host memory.
+int et_array_project_partial(float *activity, int *activity_size, float *sinogram, int *sinogram_size,
+                             float *partialsum, int *partialsum_size, float *cameras, int *cameras_size, float *psf,
+                             int *psf_size, float *attenuation, int *attenuation_size, float background,
+                             float *background_image, float background_attenuation, int GPU,
+                             int truncate_negative_values, int do_rotate_partial)
+{
+    (void)GPU;
+    (void)sinogram_size;
+    const bool no_act = activity == nullptr || is_absent(activity_size);
+    const bool no_mu = attenuation == nullptr || is_absent(attenuation_size);
+    const bool no_psf = psf == nullptr || is_absent(psf_size);
+    if (no_act && no_mu) return niftyrec_error_parameters;
+    if (no_act && activity_size) {
+        // the reference reports the attenuation size through activity_size (:1578-1582)
+        activity_size[0] = attenuation_size[0];
+        activity_size[1] = attenuation_size[1];
+        activity_size[2] = attenuation_size[2];
+    }
+    const int *vs = no_act ? attenuation_size : activity_size;
+    const int nx = vs[0], ny = vs[1], nz = vs[2];
+    if (nx <= 0 || ny <= 0 || nz <= 0) return niftyrec_error_parameters;
+    if (!no_act && !no_mu &&
+        (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
+        return niftyrec_error_parameters;
+    const int V = cameras_size[0], cols = cameras_size[1];
+    if (!(cols == 1 || cols == 3 || cols == 6) || V <= 0) return niftyrec_error_parameters;
+    int h = 0;
+    if (!no_psf) {
+        h = psf_size[0];
+        if (h % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != nx || psf_size[0] != psf_size[1])
+            return niftyrec_error_parameters;
+    }
+    if (partialsum_size && (partialsum_size[0] != nx || partialsum_size[1] != ny || partialsum_size[2] != nz ||
+                            partialsum_size[3] != V))
+        return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        const int dims[3] = {nx, ny, nz};
+        Cameras cams;
+        int rc = expand_cameras(cameras, V, cols, dims, cams);
+        if (rc) return rc;
+        HostCall hc;
+        hc.init();
+        const et_stream_t s = (et_stream_t)hc.st;
+        const size_t n = (size_t)nx * ny * nz, P = (size_t)nx * ny;
+        // forward and inverse (reference pull) matrices of every view
+        std::vector<float> mats((size_t)V * 24);
+        for (int v = 0; v < V; v++) {
+            float M[16];
+            rotation_matrix(M, cams.ang[v], cams.ang[V + v], cams.ang[2 * V + v], cams.ctr[v], cams.ctr[V + v],
+                            cams.ctr[2 * V + v], 0);
+            std::memcpy(&mats[(size_t)24 * v], M, sizeof(float) * 12);
+            rotation_matrix(M, -cams.ang[v], -cams.ang[V + v], -cams.ang[2 * V + v], cams.ctr[v], cams.ctr[V + v],
+                            cams.ctr[2 * V + v], 5);
+            std::memcpy(&mats[(size_t)24 * v + 12], M, sizeof(float) * 12);
+        }
+        const float *d_M = hc.upload(mats.data(), mats.size());
+        const float *d_act = no_act ? nullptr : hc.upload(activity, n);
+        const float *d_mu = no_mu ? nullptr : hc.upload(attenuation, n);
+        const float *d_bgimg = background_image ? hc.upload(background_image, P) : nullptr;
+        const float *d_sep = nullptr;
+        if (h > 0) {
+            const int c = (h - 1) / 2;
+            std::vector<float> sep((size_t)nz * 2 * h);
+            for (int z = 0; z < nz; z++) {
+                const float *pl = psf + (size_t)h * h * z;
+                for (int i = 0; i < h; i++) {
+                    sep[(size_t)2 * h * z + i] = pl[c * h + i];
+                    sep[(size_t)2 * h * z + h + i] = pl[c + i * h] / pl[c * h + c];
+                }
+            }
+            d_sep = hc.upload(sep.data(), sep.size());
+        }
+        float *rot_act = no_act ? nullptr : hc.alloc(n);
+        float *tmp = h > 0 ? hc.alloc(n) : nullptr;
+        float *rot_mu = no_mu ? nullptr : hc.alloc(n);
+        float *part = hc.alloc(n);
+        float *pout = do_rotate_partial ? hc.alloc(n) : part;
+        float *d_sino = hc.alloc(P * V);
+        for (int v = 0; v < V; v++) {
+            const float *Mf = d_M + (size_t)24 * v, *Mi = Mf + 12;
+            if (d_act) {
+                check_launch(et_launch_resample_general(d_act, nx, ny, nz, Mf, background, rot_act, 0, 0, 0, s));
+                if (d_sep) {
+                    check_launch(et_launch_conv_axis(rot_act, tmp, nx, ny, nz, 1, nx, (long)P, 0, 0, d_sep, h, 0, s));
+                    check_launch(et_launch_conv_axis(tmp, rot_act, nx, ny, nz, 1, nx, (long)P, 0, 1, d_sep, h, 0, s));
+                }
+            }
+            if (d_mu)
+                check_launch(
+                    et_launch_resample_general(d_mu, nx, ny, nz, Mf, background_attenuation, rot_mu, 0, 0, 0, s));
+            check_launch(et_launch_partial_line(rot_act, rot_mu, d_bgimg, background, nx, ny, nz, d_sino + P * v,
+                                                part, s));
+            if (do_rotate_partial)
+                check_launch(et_launch_resample_general(part, nx, ny, nz, Mi, background, pout, 0, 0, 0, s));
+            hc.download(partialsum + n * v, pout, n);
+        }
+        if (truncate_negative_values) check_launch(et_launch_truncate(d_sino, P * V, s));
+        hc.download(sinogram, d_sino, P * V);
+        return niftyrec_success;
+    });
+}
+
 int et_array_convolve(float *image, int *image_size, float *out, int *out_size, float *psf, int *psf_size, int GPU)
 {
     (void)GPU;

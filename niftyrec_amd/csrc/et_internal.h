@@ -158,6 +158,10 @@ int et_launch_sweep_group(int mode, int R, int has_mu, int nthr, const EtSweepAr
 int et_launch_pull_group(int epilogue, const EtPullArgs *args, const void *tab, int n, int blocks,
                          et_stream_t stream);
 int et_launch_ratio_group(const EtRatioArgs *args, const void *tab, int n, int blocks, et_stream_t stream);
+// Partial line integrals of one rotated view (ABI layouts): part[z] = sum of
+// what lies beyond plane z, sino = the whole ray; act or mu may be null.
+int et_launch_partial_line(const float *act, const float *mu, const float *bgimg, float bg, int nx, int ny, int nz,
+                           float *sino, float *part, et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).
 int et_sweep_slab_rows(int mode, int R);
 // Sweep kernel variants compiled in (for diagnostics).

@@ -1416,6 +1416,33 @@ __global__ void __launch_bounds__(256) k_gather_planes(float *__restrict__ dst, 
 
 // OSEM ratio from partial projections, reference _et_array_interface.cpp:1098-1108:
 // p = max(sum_d part[d], 0); r = p < eps ? y / eps : y / p.
+// Partial line integrals of one view in the rotated frame (reference
+// emission-lib_gpu/_et_line_integral_attenuated_gpu_kernels.cu:57-103): along
+// each ray from the far end, part[z] = S, then S = (S + act[z]) e^{-mu[z]}
+// (act or mu may be absent), S starting at bg (times bgimg); sino = final S.
+// ABI layouts; one thread per detector pixel (coalesced over x).
+__global__ void __launch_bounds__(256) k_partial_line(const float *__restrict__ act, const float *__restrict__ mu,
+                                                     const float *__restrict__ bgimg, float bg, int nx, int ny,
+                                                     int nz, float *__restrict__ sino, float *__restrict__ part)
+{
+    const size_t P = (size_t)nx * ny;
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= P) return;
+    float s = bg;
+    if (bgimg) s = s * bgimg[r];
+    for (int z = nz - 1; z >= 0; z--) {
+        const size_t i = r + P * z;
+        part[i] = s;
+        if (act && mu)
+            s = (s + act[i]) * exp_neg(mu[i]);
+        else if (act)
+            s = s + act[i];
+        else
+            s = s * exp_neg(mu[i]);
+    }
+    sino[r] = s;
+}
+
 // Grouped MLEM ratio over several problems (1-D grid; tab entry j =
 // {first block, blocks, 1, 0}).
 __global__ void __launch_bounds__(256) k_ratio_parts_group(const EtRatioArgs *__restrict__ args,
@@ -1614,6 +1641,16 @@ int et_launch_pull_group(int epilogue, const EtPullArgs *args, const void *tab, 
         hipLaunchKernelGGL((k_pull_group<false, ET_EPI_YF_STORE>), dim3(blocks), dim3(256), 0, st, args, t, n);
     else
         return (int)hipErrorInvalidValue;
+    return (int)hipGetLastError();
+}
+
+int et_launch_partial_line(const float *act, const float *mu, const float *bgimg, float bg, int nx, int ny, int nz,
+                           float *sino, float *part, et_stream_t stream)
+{
+    if (!act && !mu) return (int)hipErrorInvalidValue;
+    const size_t P = (size_t)nx * ny;
+    hipLaunchKernelGGL(k_partial_line, dim3(nblk(P, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), act,
+                       mu, bgimg, bg, nx, ny, nz, sino, part);
     return (int)hipGetLastError();
 }
 

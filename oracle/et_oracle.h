@@ -51,7 +51,12 @@
                             const REAL *psf, int h, REAL bg, REAL bga, REAL *grad);       \
     int orc_mlem_step_##S(REAL *act, const REAL *sino, const REAL *mu, int nx, int ny,    \
                           int nz, const REAL *cams, const REAL *centers, int V,           \
-                          const REAL *psf, int h, REAL bg, REAL bga, REAL eps);
+                          const REAL *psf, int h, REAL bg, REAL bga, REAL eps);           \
+    int orc_project_partial_##S(const REAL *act, const REAL *mu, int nx, int ny, int nz,  \
+                                const REAL *cams, const REAL *centers, int V,             \
+                                const REAL *psf, int h, REAL bg, const REAL *bgimg,       \
+                                REAL bga, int truncate, int rotate_partial, REAL *sino,   \
+                                REAL *partial);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

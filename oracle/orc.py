@@ -200,3 +200,20 @@ def osem_subset(V, n_sub):
     idx = np.zeros(n_sub, dtype=np.int32)
     lib().orc_osem_subset(V, n_sub, _ptr(idx, ctypes.c_int))
     return idx
+
+
+def project_partial(act, mu, cameras, psf=None, bg=0.0, bgimg=None, bga=0.0, truncate=1, rotate_partial=1,
+                    centers=None, dtype=np.float32):
+    """(sinogram (V, ny, nx), partial sums (V, nz, ny, nx)); see orc_project_partial."""
+    ref = act if act is not None else mu
+    act, mu, psf, bgimg = _prep(dtype, act, mu, psf, bgimg)
+    nz, ny, nx = ref.shape
+    cams, centers, V = _geom(ref.shape, cameras, centers, dtype)
+    sino = np.zeros((V, ny, nx), dtype=dtype)
+    part = np.zeros((V, nz, ny, nx), dtype=dtype)
+    ct = _ctype(dtype)
+    h = 0 if psf is None else psf.shape[-1]
+    getattr(lib(), "orc_project_partial_" + _suffix(dtype))(
+        _ptr(act, ct), _ptr(mu, ct), nx, ny, nz, _ptr(cams, ct), _ptr(centers, ct), V, _ptr(psf, ct), h,
+        ct(bg), _ptr(bgimg, ct), ct(bga), int(truncate), int(rotate_partial), _ptr(sino, ct), _ptr(part, ct))
+    return sino, part

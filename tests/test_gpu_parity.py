@@ -251,3 +251,28 @@ def test_spect256_full_size_views():
     e = rel_l2(g, ref)
     print("spect256 exact mu gradient (4 views): relL2 %.2e vs oracle" % e)
     assert e <= RTOL_L2
+
+
+@pytest.mark.parametrize("rotate", [1, 0])
+@pytest.mark.parametrize("K, use_mu, bg", [(5, True, 0.0), (0, True, 0.3), (3, False, 0.2)])
+def test_project_partial(rotate, K, use_mu, bg):
+    """et_array_project_partial against the oracle's restatement (the reference
+    has only a CUDA implementation, whose texture filtering is not pinned)."""
+    from niftyrec_amd import api
+    act, mu, cams, psf = _case(shape=(20, 12, 20), V=5, K=K, seed=30)
+    mu = mu if use_mu else None
+    img = (1.0 + np.random.default_rng(31).random((12, 20))).astype(np.float32) if bg else None
+    sino, part = api.project_partial(act, cams, psf, mu, background=bg, background_image=img, rotate_partial=rotate)
+    rs, rp = orc.project_partial(act, mu, cams, psf, bg=bg, bgimg=img, rotate_partial=rotate)
+    rs64, rp64 = orc.project_partial(act, mu, cams, psf, bg=bg, bgimg=img, rotate_partial=rotate, dtype=np.float64)
+    _report("partial sinogram K=%d mu=%s rotate=%d" % (K, use_mu, rotate), sino, rs, rs64)
+    _report("partial sums K=%d mu=%s rotate=%d" % (K, use_mu, rotate), part, rp, rp64)
+
+
+def test_project_partial_attenuation_only():
+    from niftyrec_amd import api
+    _, mu, cams, _ = _case(shape=(16, 10, 16), V=3, K=0, seed=32)
+    sino, part = api.project_partial(None, cams, None, mu, background=1.5)
+    rs, rp = orc.project_partial(None, mu, cams, None, bg=1.5)
+    _report("partial (attenuation only) sinogram", sino, rs)
+    _report("partial (attenuation only) sums", part, rp)

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 from scipy.signal import convolve2d
 
+from niftyrec_amd import phantoms
 from oracle import orc
 
 
@@ -236,3 +237,48 @@ def test_osem_subset_draw_matches_formula():
         R = np.float32(base + np.float32(fine / np.float32(2.0 ** 31)))
         expect.append(min(int(np.float32(40) * R), 39))
     assert list(idx) == expect
+
+
+def test_partial_projection_closed_form():
+    """Uniform activity a and attenuation mu at theta = 0, no PSF: the partial
+    sum ahead of plane z is bg e^{-(L-z)mu} + a sum_{j=1}^{L-z} e^{-j mu}
+    (L = nz - 1), and the final sum is the inclusive projection."""
+    nz, ny, nx, a, m, bg = 9, 3, 4, 1.5, 0.2, 0.7
+    act = np.full((nz, ny, nx), a)
+    mu = np.full((nz, ny, nx), m)
+    sino, part = orc.project_partial(act, mu, np.zeros(1), bg=bg, rotate_partial=0, truncate=0,
+                                     dtype=np.float64)
+    L = nz - 1
+    for z in range(nz):
+        want = bg * np.exp(-(L - z) * m) + a * sum(np.exp(-j * m) for j in range(1, L - z + 1))
+        assert np.allclose(part[0, z], want, rtol=1e-13)
+    assert np.allclose(sino[0], bg * np.exp(-nz * m) + a * sum(np.exp(-k * m) for k in range(1, nz + 1)),
+                       rtol=1e-13)
+
+
+def test_partial_projection_final_sum_is_the_projection():
+    """The recursion's final sum equals the projection (exp of the inclusive
+    sum) for a general rotation, PSF and attenuation, in double precision;
+    at theta = 0 the image-frame partial sums equal the rotated-frame ones."""
+    rng = np.random.default_rng(21)
+    act = rng.random((10, 6, 10))
+    mu = 0.1 * rng.random((10, 6, 10))
+    psf = phantoms.gaussian_psf(10, 3, 0.5, 1.0)
+    cams = np.array([0.3, 1.9, 4.0])
+    sino, _ = orc.project_partial(act, mu, cams, psf, bg=0.4, truncate=0, dtype=np.float64)
+    ref = orc.project(act, mu, cams, psf, bg=0.4, truncate=0, dtype=np.float64)
+    assert np.allclose(sino, ref, rtol=1e-12, atol=1e-14)
+    _, p_rot = orc.project_partial(act, mu, np.zeros(1), psf, bg=0.4, rotate_partial=1, dtype=np.float64)
+    _, p_img = orc.project_partial(act, mu, np.zeros(1), psf, bg=0.4, rotate_partial=0, dtype=np.float64)
+    assert np.array_equal(p_rot, p_img)
+
+
+def test_partial_projection_background_image_and_no_activity():
+    """Without activity the partial sum ahead of z is bg*img e^{-sum_{k>z} mu_k}."""
+    rng = np.random.default_rng(22)
+    mu = 0.2 * rng.random((7, 4, 5))
+    img = 1.0 + rng.random((4, 5))
+    _, part = orc.project_partial(None, mu, np.zeros(1), bg=2.0, bgimg=img, rotate_partial=0, dtype=np.float64)
+    for z in range(7):
+        want = 2.0 * img * np.exp(-mu[z + 1:].sum(axis=0))
+        assert np.allclose(part[0, z], want, rtol=1e-13)
