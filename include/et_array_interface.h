@@ -206,7 +206,8 @@ int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *
 /* Ordered-subsets EM on the device: subset s holds views s, s+S, s+2S, ...;
  * each subset update is the reference MLEM update restricted to its views.
  * act is updated in place.  The sensitivity of every subset, the attenuation
- * prefixes and the depth chunking (timed on the first call unless fixed with
+ * factors e^{-A} of every view (4 bytes per voxel and view of device memory)
+ * and the depth chunking (timed on the first call unless fixed with
  * et_plan_set_tuning) are cached in the plan, keyed on (subsets, mu pointer,
  * eps); a caller that rewrites mu in place calls et_plan_set_tuning to drop
  * the cache. */

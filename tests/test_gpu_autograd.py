@@ -141,3 +141,21 @@ def test_plan_osem_matches_sequential_mlem_on_subsets():
     e = rel_l2(X.cpu().numpy(), x)
     print("plan osem (4 subsets x 2 iterations) relL2 vs oracle %.2e" % e)
     assert e <= 1e-5
+
+
+@pytest.mark.parametrize("shape, V, K", [((96, 20, 96), 12, 7), ((160, 9, 160), 8, 3)])
+def test_plan_osem_wide_volumes(shape, V, K):
+    """OSEM through the plan at widths that leave the last wave of a block
+    partly idle (96: 2 waves, 160: 3 waves) and ragged axial sizes: cached
+    attenuation factors in the padded layout, coalesced stores, tuned chunks."""
+    act, mu, cams, psf = _case(shape=shape, V=V, K=K, seed=12)
+    plan = _plan(act.shape, cams, psf)
+    y = orc.project(act, mu, cams, psf)
+    X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
+    plan.osem(X, _t(y), _t(mu), subsets=2, iterations=1)
+    x = np.ones_like(act)
+    for s in range(2):
+        x = orc.mlem_step(x, y[s::2], mu, cams[s::2], psf)
+    e = rel_l2(X.cpu().numpy(), x)
+    print("plan osem %s: relL2 vs oracle %.2e" % (shape, e))
+    assert e <= 1e-5
