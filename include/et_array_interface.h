@@ -185,7 +185,10 @@ int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y
  * Calls on one plan must not overlap; distinct plans are independent. */
 typedef struct et_plan et_plan;
 
-/* cameras as in Part 1 (host memory, camera_cols 1, 3 or 6); psf host memory
+/* Volumes up to 512 voxels wide (nx; the projection kernels hold one thread
+ * per rotated column); wider ones are refused with status 2 by the plan and
+ * by the projection, backprojection, gradient and MLEM/OSEM entry points.
+ * cameras as in Part 1 (host memory, camera_cols 1, 3 or 6); psf host memory
  * [psf_h, psf_h, nz] or NULL.  For camera_cols != 6 the rotation centres are
  * ((nx-1)/2, (ny-1)/2, (nz-1)/2). */
 int et_plan_create(et_plan **plan, int nx, int ny, int nz, const float *cameras,

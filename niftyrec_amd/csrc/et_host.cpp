@@ -1145,7 +1145,7 @@ int project_host(float *activity, int *activity_size, float *sinogram, int *sino
         return niftyrec_error_parameters;
     if (sinogram_size && (sinogram_size[0] != nx || sinogram_size[1] != ny || sinogram_size[2] != V))
         return niftyrec_error_parameters;
-    if (nx > 1024) return niftyrec_error_parameters;
+    if (nx > ET_MAX_NX) return niftyrec_error_parameters;
     int st = select_device();
     if (st) return st;
     return guarded([&]() -> int {
@@ -1188,7 +1188,7 @@ int backproject_host(float *sino, int *sino_size, float *bkpr, int *bkpr_size, f
     }
     if (!no_mu && (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
         return niftyrec_error_parameters;
-    if (nx > 1024) return niftyrec_error_parameters;
+    if (nx > ET_MAX_NX) return niftyrec_error_parameters;
     int st = select_device();
     if (st) return st;
     return guarded([&]() -> int {
@@ -1218,6 +1218,7 @@ int backproject_host(float *sino, int *sino_size, float *bkpr, int *bkpr_size, f
 int em_step_host(float *activity, int size_x, int size_y, const float *sino, int V, const float *cams3,
                  const float *attenuation, const float *psf, int psf_h, float bg, float bga, float eps)
 {
+    if (size_x > ET_MAX_NX) return niftyrec_error_parameters;
     int st = select_device();
     if (st) return st;
     return guarded([&]() -> int {
@@ -1633,7 +1634,7 @@ int et_array_gradient_attenuation(float *sino, int *sino_size, float *activity, 
     if (attenuation_size && !is_absent(attenuation_size) &&
         (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
         return niftyrec_error_parameters;
-    if (nx > 1024) return niftyrec_error_parameters;
+    if (nx > ET_MAX_NX) return niftyrec_error_parameters;
     int st = select_device();
     if (st) return st;
     return guarded([&]() -> int {
@@ -1899,7 +1900,7 @@ int et_array_mlem_step(float *activity_data, int size_x, int size_y, float *sino
     (void)use_gpu;
     if (size_x <= 0 || size_y <= 0 || n_cameras <= 0) return niftyrec_error_parameters;
     if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
-    if (size_x > 1024) return niftyrec_error_parameters;
+    if (size_x > ET_MAX_NX) return niftyrec_error_parameters;
     return em_step_host(activity_data, size_x, size_y, sinogram_data, n_cameras, cameras_data,
                         use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr,
                         (int)psf_size_x, background, background_attenuation, epsilon);
@@ -1974,7 +1975,7 @@ int et_plan_create(et_plan **plan, int nx, int ny, int nz, const float *cameras,
 {
     if (!plan) return niftyrec_error_parameters;
     *plan = nullptr;
-    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > 1024 || n_cameras <= 0) return niftyrec_error_parameters;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > ET_MAX_NX || n_cameras <= 0) return niftyrec_error_parameters;
     if (psf && (psf_h % 2 != 1 || nz != nx)) return niftyrec_error_parameters;
     int st = select_device();
     if (st) return st;

@@ -23,6 +23,7 @@
 #define ET_GY 16     // guard rows of the YF layout (>= roundup(R, 4))
 #define ET_RMAX 10   // largest PSF radius handled inside the sweep kernels (K <= 21)
 #define ET_VIEW_REC 40  // floats per view record (see EtViewRec)
+#define ET_MAX_NX 512   // widest volume: sweep blocks hold one thread per rotated column (512 threads)
 
 // Per-view record in device memory (ET_VIEW_REC floats):
 //   [0..11]  forward 3x4 matrix M (row-major): sample position = M (x,y,z,1)

@@ -110,6 +110,13 @@ def test_parameter_errors_before_device():
     assert refabi.project(A, act, None, cams, np.ones((5, 3, 3), np.float32))[0] == 2
     # backprojection: sinogram views != cameras
     assert refabi.backproject(A, np.zeros((3, 6, 8), np.float32), None, cams)[0] == 2
+    # wider than the projection kernels take (one thread per rotated column, 512)
+    wide = np.ones((4, 2, 520), np.float32)
+    assert refabi.project(A, wide, None, cams)[0] == 2
+    assert refabi.mlem_step(A, wide, np.ones((4, 2, 520), np.float32), None, cams)[0] == 2
+    plan = ctypes.c_void_p()
+    assert A.et_plan_create(ctypes.byref(plan), 520, 2, 520, cams.ctypes.data_as(_lib.fp), 4, 1, None, 0,
+                            ctypes.c_float(0), ctypes.c_float(0)) == 2
 
 
 def test_no_device_is_a_status_not_an_exit():
