@@ -159,3 +159,23 @@ def test_plan_osem_wide_volumes(shape, V, K):
     e = rel_l2(X.cpu().numpy(), x)
     print("plan osem %s: relL2 vs oracle %.2e" % (shape, e))
     assert e <= 1e-5
+
+
+def test_plan_osem_general_path():
+    """A plan with a non-zero background takes the materialising path: OSEM
+    with one subset equals the oracle's MLEM steps; more subsets are refused
+    with a parameter status (the general path has no subset caches)."""
+    from niftyrec_amd import _lib
+    act, mu, cams, psf = _case(seed=13, V=8)
+    plan = _plan(act.shape, cams, psf, bg=0.2)
+    y = orc.project(act, mu, cams, psf, bg=0.2)
+    X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
+    plan.osem(X, _t(y), _t(mu), subsets=1, iterations=2)
+    x = np.ones_like(act)
+    for _ in range(2):
+        x = orc.mlem_step(x, y, mu, cams, psf, bg=0.2)
+    e = rel_l2(X.cpu().numpy(), x)
+    print("plan osem, general path: relL2 vs oracle %.2e" % e)
+    assert e <= 1e-5
+    with pytest.raises(_lib.NiftyRecError):
+        plan.osem(X, _t(y), _t(mu), subsets=2, iterations=1)

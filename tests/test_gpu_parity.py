@@ -152,6 +152,21 @@ def test_mlem_step(lib):
             orc.mlem_step(x0, y, mu, cams, psf, dtype=np.float64))
 
 
+@pytest.mark.parametrize("bg, three_axis", [(0.3, False), (0.0, True)])
+def test_mlem_step_general_path(lib, bg, three_axis):
+    """MLEM step on the materialising path: background != 0 (the reference
+    resampler's background rule) or rotations about all three axes."""
+    act, mu, cams, psf = _case(seed=17, V=8)
+    if three_axis:
+        cams = np.random.default_rng(17).uniform(-0.5, 0.5, (3, 8)).astype(np.float32)
+    y = orc.project(act, mu, cams, psf, bg=bg)
+    x0 = np.ones_like(act)
+    st, got = refabi.mlem_step(lib, x0, y, mu, cams, psf, bg=bg)
+    assert st == 0
+    _report("mlem step (bg %.1f, three-axis %s)" % (bg, three_axis), got,
+            orc.mlem_step(x0, y, mu, cams, psf, bg=bg), orc.mlem_step(x0, y, mu, cams, psf, bg=bg, dtype=np.float64))
+
+
 def test_osem_step_random_subset(lib):
     """et_array_osem_step draws its subset from glibc rand(): seed it, replay the
     draw in the oracle and compare with the MLEM update on that subset."""
