@@ -138,8 +138,6 @@ int et_launch_fill(float *buf, size_t n, float value, et_stream_t stream);
 int et_launch_clamp_min(float *buf, size_t n, float lo, et_stream_t stream);
 int et_launch_yf_clamp_min(float *yf, int nx, int ny, int nz, int yp, int gy, float lo,
                            et_stream_t stream);
-int et_launch_osem_yf_update(float *x_yf, const float *b_yf, const float *s_yf, int nx, int ny,
-                             int nz, int yp, int gy, et_stream_t stream);
 int et_launch_em_update_flat(float *x, const float *b, const float *s, size_t n, et_stream_t stream);
 int et_launch_gather_planes(float *dst, const float *src, const int *order, int V, size_t P,
                             et_stream_t stream);
@@ -165,5 +163,3 @@ int et_launch_partial_line(const float *act, const float *mu, const float *bgimg
                            float *sino, float *part, et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).
 int et_sweep_slab_rows(int mode, int R);
-// Sweep kernel variants compiled in (for diagnostics).
-int et_sweep_variant_count(void);

@@ -1385,15 +1385,6 @@ struct ClampMinOp {
         if (p[k] < lo) p[k] = lo;
     }
 };
-struct OsemOp {
-    float *x;
-    const float *b, *s;
-    __device__ void operator()(size_t k) const
-    {
-        float bb = b[k] < 0.f ? 0.f : b[k];
-        x[k] = x[k] * (bb / s[k]);
-    }
-};
 
 __global__ void __launch_bounds__(256) k_em_update_flat(float *__restrict__ x, const float *__restrict__ b,
                                                         const float *__restrict__ s, size_t n)
@@ -1681,7 +1672,6 @@ int et_launch_sweep(int mode, int R, int has_mu, int has_act, const EtSweepArgs 
     }
 }
 
-int et_sweep_variant_count(void) { return (ET_RMAX + 1) * 6 + 1; }
 
 int et_launch_pull(const EtPullArgs &a, et_stream_t stream)
 {
@@ -1787,15 +1777,6 @@ int et_launch_yf_clamp_min(float *yf, int nx, int ny, int nz, int yp, int gy, fl
     const size_t n = (size_t)nx * ny * nz;
     hipLaunchKernelGGL((k_yf_map<ClampMinOp>), dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                        nx, ny, nz, yp, gy, ClampMinOp{yf, lo});
-    return (int)hipGetLastError();
-}
-
-int et_launch_osem_yf_update(float *x_yf, const float *b_yf, const float *s_yf, int nx, int ny, int nz, int yp, int gy,
-                             et_stream_t stream)
-{
-    const size_t n = (size_t)nx * ny * nz;
-    hipLaunchKernelGGL((k_yf_map<OsemOp>), dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                       nx, ny, nz, yp, gy, OsemOp{x_yf, b_yf, s_yf});
     return (int)hipGetLastError();
 }
 
