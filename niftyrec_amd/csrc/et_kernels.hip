@@ -69,6 +69,31 @@ __device__ __forceinline__ float exp_neg(float A)
     return __builtin_amdgcn_exp2f(x) * fmaf(r, LN2, 1.f);
 }
 
+// Loads and stores through the global address space.  Pointers read from an
+// argument record in memory (the grouped launches) are generic to the
+// compiler, which would otherwise emit flat accesses.
+// (The qualifiers apply to the device compilation; the host pass, which
+// never runs these functions, sees plain pointers.)
+#ifdef __HIP_DEVICE_COMPILE__
+#define ET_GLOBAL_AS __attribute__((address_space(1)))
+#define ET_CONST_AS __attribute__((address_space(4)))
+#else
+#define ET_GLOBAL_AS
+#define ET_CONST_AS
+#endif
+typedef ET_GLOBAL_AS const float g_cfloat;
+typedef ET_GLOBAL_AS float g_float;
+typedef ET_GLOBAL_AS const float4 g_cfloat4;
+typedef ET_GLOBAL_AS float4 g_float4;
+__device__ __forceinline__ float gload(const float *p) { return *(g_cfloat *)p; }
+__device__ __forceinline__ float4 gload4(const float *p) { return *(g_cfloat4 *)p; }
+__device__ __forceinline__ void gstore(float *p, float v) { *(g_float *)p = v; }
+__device__ __forceinline__ void gstore4(float *p, float4 v) { *(g_float4 *)p = v; }
+// Read-only, block-uniform data (view records, PSF taps, argument records)
+// read through the constant address space: scalar loads.
+typedef ET_CONST_AS const float c_cfloat;
+__device__ __forceinline__ float cload(const float *p) { return *(c_cfloat *)p; }
+
 // Bilinear sample in the (x, z) plane shared by a run of axial rows.
 struct Samp {
     int off;                        // element offset of corner (x0, z0) in a padded layout
@@ -118,12 +143,11 @@ __device__ __forceinline__ void bilinear_rows(const float *__restrict__ p00, int
     const float w[4] = {s.w00, s.w01, s.w10, s.w11};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        const float4 *q = reinterpret_cast<const float4 *>(cp[c]);
         float run[LA];
 #pragma unroll
         for (int k = 0; k < LA / 4; k++) {
             if (4 * k + 3 < SA || 4 * k >= SA + NA) continue;
-            const float4 t = q[k];
+            const float4 t = gload4(cp[c] + 4 * k);
             run[4 * k] = t.x;
             run[4 * k + 1] = t.y;
             run[4 * k + 2] = t.z;
@@ -150,7 +174,7 @@ __device__ __forceinline__ void column_pass(const float *in, const float *__rest
         } else {
             float t = 0.f;
 #pragma unroll
-            for (int j = 0; j <= 2 * R; j++) t = fmaf(in[i + j], kc[2 * R - j], t);
+            for (int j = 0; j <= 2 * R; j++) t = fmaf(in[i + j], cload(kc + 2 * R - j), t);
             yc[i] = t;
         }
     }
@@ -235,10 +259,10 @@ __device__ __forceinline__ void coop_rows(const float *__restrict__ base, int dx
             if (serve && tt < OPP && t < 64) {
                 const float *q = base + __float_as_int(tab[t]) + 4 * k;
                 const float w00 = tab[64 + t], w01 = tab[128 + t], w10 = tab[192 + t], w11 = tab[256 + t];
-                const float4 a = *reinterpret_cast<const float4 *>(q);
-                const float4 b = *reinterpret_cast<const float4 *>(q + dx);
-                const float4 c = *reinterpret_cast<const float4 *>(q + dz);
-                const float4 d = *reinterpret_cast<const float4 *>(q + dz + dx);
+                const float4 a = gload4(q);
+                const float4 b = gload4(q + dx);
+                const float4 c = gload4(q + dz);
+                const float4 d = gload4(q + dz + dx);
                 float4 v;
                 v.x = a.x * w00 + b.x * w01 + c.x * w10 + d.x * w11;
                 v.y = a.y * w00 + b.y * w01 + c.y * w10 + d.y * w11;
@@ -285,7 +309,7 @@ __device__ __forceinline__ void row_pass(float *buf, int tx, const float *yc, co
     for (int i = 0; i < TS; i++) out[i] = 0.f;
 #pragma unroll
     for (int j = 0; j <= 2 * R; j++) {
-        const float kj = kr[2 * R - j];
+        const float kj = cload(kr + 2 * R - j);
         const float4 *r4 = reinterpret_cast<const float4 *>(buf + (tx + j) * L);
 #pragma unroll
         for (int q = 0; q < TS / 4; q++) {
@@ -316,12 +340,12 @@ __device__ __forceinline__ const float *psf_plane(const float *psf, int z)
 __device__ __forceinline__ FastView load_fast_view(const float *rec)
 {
     FastView f;
-    f.m00 = rec[0];
-    f.m02 = rec[2];
-    f.m03 = rec[3];
-    f.m20 = rec[8];
-    f.m22 = rec[10];
-    f.m23 = rec[11];
+    f.m00 = cload(rec + 0);
+    f.m02 = cload(rec + 2);
+    f.m03 = cload(rec + 3);
+    f.m20 = cload(rec + 8);
+    f.m22 = cload(rec + 10);
+    f.m23 = cload(rec + 11);
     return f;
 }
 
@@ -339,13 +363,13 @@ __device__ __forceinline__ float load_source(const EtSweepArgs &a, int v, int y,
     const size_t P = (size_t)a.nx * a.ny;
     const size_t r = (size_t)y * a.nx + x;
     if (a.src == ET_SRC_ONES) return 1.f;
-    if (a.src == ET_SRC_SINO) return a.sino[(size_t)v * P + r];
+    if (a.src == ET_SRC_SINO) return gload(a.sino + (size_t)v * P + r);
     // ET_SRC_RATIO: reference _et_array_interface.cpp:1098-1108 (projection
     // truncated at 0, r = p < eps ? y/eps : y/p).
     float p = 0.f;
-    for (int d = 0; d < a.rD; d++) p += a.rpart[((size_t)d * a.V + v) * P + r];
+    for (int d = 0; d < a.rD; d++) p += gload(a.rpart + ((size_t)d * a.V + v) * P + r);
     if (p < 0.f) p = 0.f;
-    const float m = a.meas[(size_t)v * P + r];
+    const float m = gload(a.meas + (size_t)v * P + r);
     return (p < a.eps) ? m / a.eps : m / p;
 }
 
@@ -354,9 +378,9 @@ template <int TS>
 __device__ __forceinline__ void store_u(const EtSweepArgs &a, int v, int tx, int z, int y0, const float *val)
 {
     float *dst = a.u + (size_t)v * a.ustride + (size_t)a.yu * ((tx + 1) + (size_t)a.xp * (z + 1)) + y0;
-    float4 *d4 = reinterpret_cast<float4 *>(dst);
 #pragma unroll
-    for (int q = 0; q < TS / 4; q++) d4[q] = make_float4(val[4 * q], val[4 * q + 1], val[4 * q + 2], val[4 * q + 3]);
+    for (int q = 0; q < TS / 4; q++)
+        gstore4(dst + 4 * q, make_float4(val[4 * q], val[4 * q + 1], val[4 * q + 2], val[4 * q + 3]));
 }
 
 // Coalesced form of store_u for the waves of a sweep block with a PSF: the
@@ -381,8 +405,7 @@ __device__ __forceinline__ void store_u_coop(const EtSweepArgs &a, int v, int xb
         const int t = r * CPR + tl;
         const int xq = xbase + t;
         if (xq < a.nx)
-            *reinterpret_cast<float4 *>(uz + (size_t)a.yu * (xq + 1)) =
-                *reinterpret_cast<const float4 *>(stage + t * L + 4 * k);
+            gstore4(uz + (size_t)a.yu * (xq + 1), *reinterpret_cast<const float4 *>(stage + t * L + 4 * k));
     }
     wave_sync();
 }
@@ -527,7 +550,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
 #pragma unroll
         for (int i = 0; i < TS; i++) {
             const int y = c.y0 + i;
-            A[i] = (y < a.ny) ? as[(size_t)y * a.nx + c.tx] : 0.f;
+            A[i] = (y < a.ny) ? gload(as + (size_t)y * a.nx + c.tx) : 0.f;
         }
     }
 
@@ -578,11 +601,11 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             for (int i = 0; i < TS; i++) {
                 const float e = exp_neg(A[i]);
                 acc[i] = fmaf(h[i], e, acc[i]);
-                ez[i * c.nthr] = e;
+                gstore(ez + i * c.nthr, e);
             }
         } else if constexpr (HAS_MU && AM == 2) {
 #pragma unroll
-            for (int i = 0; i < TS; i++) acc[i] = fmaf(h[i], ez[i * c.nthr], acc[i]);
+            for (int i = 0; i < TS; i++) acc[i] = fmaf(h[i], gload(ez + i * c.nthr), acc[i]);
         } else {
 #pragma unroll
             for (int i = 0; i < TS; i++) acc[i] = HAS_MU ? fmaf(h[i], exp_neg(A[i]), acc[i]) : acc[i] + h[i];
@@ -609,7 +632,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             if (y < a.ny) {
                 float val = acc[i];
                 if (a.truncate && val < 0.f) val = 0.f;
-                out[(size_t)y * a.nx + c.tx] = val;
+                gstore(out + (size_t)y * a.nx + c.tx, val);
             }
         }
     }
@@ -642,7 +665,7 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
         const int y = c.y0 - R + j;
         const bool ok = c.xin && y >= 0 && y < a.ny;
         sv[j] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
-        A[j] = (HAS_MU && !ATT && ok && c.d > 0) ? a.astart[((size_t)c.v * a.D + c.d) * c.P + (size_t)y * a.nx + c.tx]
+        A[j] = (HAS_MU && !ATT && ok && c.d > 0) ? gload(a.astart + ((size_t)c.v * a.D + c.d) * c.P + (size_t)y * a.nx + c.tx)
                                                  : 0.f;
     }
 
@@ -667,7 +690,7 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
 #pragma unroll
             for (int j = 0; j < NA; j++) {
                 const int yl = min(max(c.y0 - R + j, 0), a.ny - 1);
-                rows[j] = sv[j] * ez[yl * c.nthr];
+                rows[j] = sv[j] * gload(ez + yl * c.nthr);
             }
             column_pass<TS, R>(rows, kz + K, yc);
         } else if constexpr (HAS_MU) {
@@ -691,7 +714,7 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
             for (int i = 0; i < TS; i++) {
                 float t = 0.f;
 #pragma unroll
-                for (int j = 0; j <= 2 * R; j++) t = fmaf(sv[i + j], kz[K + 2 * R - j], t);
+                for (int j = 0; j <= 2 * R; j++) t = fmaf(sv[i + j], cload(kz + K + 2 * R - j), t);
                 yc[i] = t;
             }
         }
@@ -727,7 +750,7 @@ __global__ void __launch_bounds__(512) k_project_group(const EtSweepArgs *__rest
     Blk b;
     int j;
     if (!group_block(tab, n, b, j)) return;
-    const EtSweepArgs a = args[j];
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
     project_body<TS, R, HAS_MU, COOP, AM>(a, b);
 }
 
@@ -744,7 +767,7 @@ __global__ void __launch_bounds__(512) k_bpu_group(const EtSweepArgs *__restrict
     Blk b;
     int j;
     if (!group_block(tab, n, b, j)) return;
-    const EtSweepArgs a = args[j];
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
     bpu_body<TS, R, HAS_MU, COOP, ATT>(a, b);
 }
 
@@ -883,8 +906,8 @@ __global__ void __launch_bounds__(512) k_musum(EtSweepArgs a)
 __device__ __forceinline__ void pull_ref_view(const EtPullArgs &a, const float *rec, const float *uv,
                                               int x, int z, int y0, float *tab, float *stage, float *acc)
 {
-    const float qx = pos2(rec[12], (float)x, rec[14], (float)z, rec[15]);
-    const float qz = pos2(rec[20], (float)x, rec[22], (float)z, rec[23]);
+    const float qx = pos2(cload(rec + 12), (float)x, cload(rec + 14), (float)z, cload(rec + 15));
+    const float qz = pos2(cload(rec + 20), (float)x, cload(rec + 22), (float)z, cload(rec + 23));
     const int dx = a.yu, dz = a.yu * a.xp;
     const Samp s = make_samp(qx, qz, a.nx, a.nz, dx, dz);
     float dummy[T];
@@ -899,8 +922,9 @@ __device__ __forceinline__ void pull_ref_view(const EtPullArgs &a, const float *
 __device__ __forceinline__ void pull_adj_view(const EtPullArgs &a, const float *rec, const float *uv,
                                               int x, int z, int y0, float *acc)
 {
-    const float f00 = rec[0], f02 = rec[2], f03 = rec[3], f20 = rec[8], f22 = rec[10], f23 = rec[11];
-    const float i00 = rec[24], i02 = rec[26], i20 = rec[30], i22 = rec[32];
+    const float f00 = cload(rec + 0), f02 = cload(rec + 2), f03 = cload(rec + 3), f20 = cload(rec + 8),
+                f22 = cload(rec + 10), f23 = cload(rec + 11);
+    const float i00 = cload(rec + 24), i02 = cload(rec + 26), i20 = cload(rec + 30), i22 = cload(rec + 32);
     const float rx = (float)x - f03, rz = (float)z - f23;
     const float cx = i00 * rx + i02 * rz, cz = i20 * rx + i22 * rz;
     const float hx = fabsf(i00) + fabsf(i02) + 0.01f, hz = fabsf(i20) + fabsf(i22) + 0.01f;
@@ -961,12 +985,12 @@ __device__ __forceinline__ void yf_epilogue_coop(const EtPullArgs &a, const floa
             const float4 b = *reinterpret_cast<const float4 *>(stage + t * RS + 4 * k);
             const float bv[4] = {b.x, b.y, b.z, b.w};
             float *dst = EPI == ET_EPI_OSEM ? a.x : a.out;
-            float4 *d4 = reinterpret_cast<float4 *>(dst + base);
-            const float4 o = (EPI == ET_EPI_OSEM || a.accumulate) ? *d4 : make_float4(0.f, 0.f, 0.f, 0.f);
+            float *d4 = dst + base;
+            const float4 o = (EPI == ET_EPI_OSEM || a.accumulate) ? gload4(d4) : make_float4(0.f, 0.f, 0.f, 0.f);
             const float ov[4] = {o.x, o.y, o.z, o.w};
             float sv[4] = {1.f, 1.f, 1.f, 1.f};
             if constexpr (EPI == ET_EPI_OSEM) {
-                const float4 s4 = *reinterpret_cast<const float4 *>(a.sens + base);
+                const float4 s4 = gload4(a.sens + base);
                 sv[0] = s4.x; sv[1] = s4.y; sv[2] = s4.z; sv[3] = s4.w;
             }
             float nv[4];
@@ -983,7 +1007,7 @@ __device__ __forceinline__ void yf_epilogue_coop(const EtPullArgs &a, const floa
                 }
                 nv[e] = live ? val : ov[e];
             }
-            *d4 = make_float4(nv[0], nv[1], nv[2], nv[3]);
+            gstore4(d4, make_float4(nv[0], nv[1], nv[2], nv[3]));
         }
     }
 }
@@ -1070,7 +1094,7 @@ __global__ void __launch_bounds__(256) k_pull_group(const EtPullArgs *__restrict
     Blk b;
     int j;
     if (!group_block(tab, n, b, j)) return;
-    const EtPullArgs a = args[j];
+    const EtPullArgs a = ((ET_CONST_AS const EtPullArgs *)args)[j];
     pull_body<ADJ, EPI>(a, b);
 }
 
