@@ -3,6 +3,7 @@ PSF 11x11 and attenuation, 8 ordered subsets of 15 views, on one MI355X with
 every input resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--workload spect128|batch48|joint256|hbm_copy] [--depth-chunks D]
 
 A step is one OSEM iteration: 8 subset updates, each = projection of the
 subset's 15 views -> ratio -> reference backprojection -> multiplicative
@@ -30,18 +31,26 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="spect128")
+    ap.add_argument("--depth-chunks", type=int, default=0,
+                    help="fix the sweeps' depth chunking (0: timed by the plan); profiler runs fix it "
+                         "because counter collection perturbs the timing")
     return ap.parse_args()
 
 
 # Kernel of each profiled category, as named in the rocprofv3 summaries.
-KERNEL_OF = {"project": "void k_project<", "bp_rotated_frame": "void k_bpu<", "bp_pull": "void k_pull<false, 2>"}
+# Kernel of each profiled category in the OSEM subset loop at this workload (slabs of
+# 16 rows, PSF radius 5, attenuation factors read from the cache), as named in the
+# rocprofv3 summaries.
+KERNEL_OF = {"project": "void k_project<16, 5, true, true, 2>", "bp_rotated_frame": "void k_bpu<16, 5, true, false, true>",
+             "bp_pull": "void k_pull<false, 2>"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "osem_spect128_hbm_traffic.txt")
 
 
-def measured_traffic(cat):
+def measured_traffic(cat, grid=None):
     """HBM bytes per launch of the category's kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (the subset launches: the row with
-    the most dispatches), or None when the summary lacks it."""
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary: the row of that launch grid
+    (work-items), else the row with the most dispatches; None when the summary
+    lacks the kernel."""
     prefix = KERNEL_OF.get(cat)
     if prefix is None or not os.path.exists(TRAFFIC_FILE):
         return None
@@ -50,10 +59,20 @@ def measured_traffic(cat):
         if not ln.startswith(prefix):
             continue
         f = ln.split()
-        n, mb = int(f[-4]), float(f[-1])
+        g, n, mb = int(f[-5]), int(f[-4]), float(f[-1])
+        if grid is not None and g == grid:
+            return mb * 1e6
         if best is None or n > best[0]:
             best = (n, mb)
     return None if best is None else best[1] * 1e6
+
+
+def sweep_grid(cat, nx, ny, views, depth_chunks):
+    """Work-items of one subset launch of a sweep kernel: axial slabs of 16
+    rows x views x depth chunks blocks of nx rounded up to 64 threads."""
+    if cat not in ("project", "bp_rotated_frame"):
+        return None
+    return ((ny + 15) // 16) * views * depth_chunks * ((nx + 63) // 64 * 64)
 
 
 def cpu_baseline(w, subsets):
@@ -225,6 +244,28 @@ def main_joint(args):
     print(json.dumps(line), flush=True)
 
 
+def main_copy(args):
+    """Device copy of 256 MiB (torch clone), the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters: a copy of known size reads and writes 256 MiB."""
+    import torch
+
+    n = 64 * 1024 * 1024
+    x = torch.ones(n, device="cuda:0")
+    for _ in range(max(args.warmup, 1)):
+        x.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        x.clone()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    line = {"metric": "device copy bandwidth (read + write)", "value": 2 * 4 * n / dt / 1e9, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "hbm_copy: 256 MiB torch clone (counter calibration)"}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -235,6 +276,8 @@ def main():
         return main_batch(args)
     if args.workload == "joint256":
         return main_joint(args)
+    if args.workload == "hbm_copy":
+        return main_copy(args)
     import torch
     from niftyrec_amd import api, phantoms
 
@@ -247,6 +290,8 @@ def main():
     A = torch.from_numpy(act).to(dev)
     M = torch.from_numpy(mu).to(dev)
     plan = api.Plan(act.shape, cams, psf)
+    if args.depth_chunks > 0:
+        plan.set_tuning(args.depth_chunks, 0)
     noiseless = plan.project(A, M).cpu().numpy()
     Y = torch.from_numpy(phantoms.poisson_data(noiseless, w["counts"], seed=1)).to(dev)
     X = torch.ones_like(A)
@@ -254,6 +299,8 @@ def main():
     # warmup (first call builds the cached sensitivities and attenuation factors)
     plan.osem(X, Y, M, subsets=S, iterations=max(args.warmup, 1))
     torch.cuda.synchronize()
+
+    dchunks = plan.tuning()  # depth chunks chosen by the plan's timed tuning (projection, backprojection)
 
     # untimed pass with every launch timed: per-kernel split, dominant kernel
     plan.profile(True)
@@ -303,8 +350,10 @@ def main():
         "gvoxel_views_per_s": V * n / (ms * 1e-3) / 1e9,
         "hbm_frac_of_peak": bytes_iter / (ms * 1e-3) / HBM_PEAK,
         "roofline": {"bound": "hbm", "kernel": cat, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(cat),
-                     "avg_launch_ms": tot_ms / launches, "algorithmic_bytes_per_launch": tot_bytes / launches},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": measured_traffic(cat, sweep_grid(cat, nx, ny, V // S, dchunks[cat == "bp_rotated_frame"])),
+                     "avg_launch_ms": tot_ms / launches, "algorithmic_bytes_per_launch": tot_bytes / launches,
+                     "depth_chunks": dchunks[cat == "bp_rotated_frame"]},
         "kernel_ms_per_step": {k: v[1] / n_split for k, v in split.items() if v[0]},
         "kernel_split_note": "per-kernel times from an untimed pass with every launch timed; the timed "
                              "region times only the dominant kernel",

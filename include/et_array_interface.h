@@ -218,6 +218,9 @@ int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, 
                  int iterations, float epsilon, void *stream);
 /* Tuning: depth chunks per ray (0 = automatic), views per backprojection batch (0 = automatic). */
 int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch);
+/* Depth chunks per ray of the cached OSEM data: projection and rotated-frame
+ * backprojection sweeps (0 and 0 before the first et_plan_osem). */
+int et_plan_get_tuning(const et_plan *plan, int *depth_chunks_project, int *depth_chunks_backproject);
 /* Kernel timing with HIP events on the plan's launches (category ids below):
  * et_plan_profile times every category (enable != 0) or none; _select times
  * the categories whose bits are set in `categories` (bit ET_PROF_*), so that

@@ -276,6 +276,13 @@ class Plan:
     def set_tuning(self, depth_chunks=0, view_batch=0):
         _lib.check(self._L.et_plan_set_tuning(self._h, int(depth_chunks), int(view_batch)), "et_plan_set_tuning")
 
+    def tuning(self):
+        """(projection, backprojection) depth chunks of the cached OSEM data
+        ((0, 0) before the first osem call)."""
+        dp, db = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._L.et_plan_get_tuning(self._h, ctypes.byref(dp), ctypes.byref(db)), "et_plan_get_tuning")
+        return dp.value, db.value
+
     def profile(self, enable=True, categories=None):
         """Time the plan's launches with HIP events: every category, or only the
         named ones (names of _lib.PROF_NAMES); resets the figures."""

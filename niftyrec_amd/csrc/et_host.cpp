@@ -2050,6 +2050,15 @@ int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch_)
     return niftyrec_success;
 }
 
+int et_plan_get_tuning(const et_plan *plan, int *depth_chunks_project, int *depth_chunks_backproject)
+{
+    if (!plan || !depth_chunks_project || !depth_chunks_backproject) return niftyrec_error_parameters;
+    const bool cached = plan->osem_S > 0;
+    *depth_chunks_project = cached ? plan->osem_Dp : 0;
+    *depth_chunks_backproject = cached ? plan->osem_Db : 0;
+    return niftyrec_success;
+}
+
 int et_plan_profile(et_plan *plan, int enable)
 {
     return et_plan_profile_select(plan, enable ? (1u << ET_PROF_COUNT) - 1u : 0u);

@@ -143,6 +143,32 @@ def test_plan_osem_matches_sequential_mlem_on_subsets():
     assert e <= 1e-5
 
 
+def test_plan_osem_fixed_depth_chunks():
+    """et_plan_set_tuning fixes the sweeps' depth chunking and et_plan_get_tuning
+    reports the chunking of the cached OSEM data ((0, 0) before the first OSEM
+    call, the timed choice after it); every chunking gives the oracle's result."""
+    act, mu, cams, psf = _case(seed=14, V=8)
+    plan = _plan(act.shape, cams, psf)
+    assert plan.tuning() == (0, 0)
+    y = orc.project(act, mu, cams, psf)
+    x = np.ones_like(act)
+    for s in range(2):
+        x = orc.mlem_step(x, y[s::2], mu, cams[s::2], psf)
+    for D in (0, 1, 3):
+        plan.set_tuning(D, 0)
+        assert plan.tuning() == (0, 0)      # the cache is dropped until the next OSEM call
+        X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
+        plan.osem(X, _t(y), _t(mu), subsets=2, iterations=1)
+        dp, db = plan.tuning()
+        if D:
+            assert (dp, db) == (D, D)
+        else:
+            assert dp >= 1 and db >= 1
+        e = rel_l2(X.cpu().numpy(), x)
+        print("plan osem, depth chunks (%d, %d): relL2 vs oracle %.2e" % (dp, db, e))
+        assert e <= 1e-5
+
+
 @pytest.mark.parametrize("shape, V, K", [((96, 20, 96), 12, 7), ((160, 9, 160), 8, 3)])
 def test_plan_osem_wide_volumes(shape, V, K):
     """OSEM through the plan at widths that leave the last wave of a block
