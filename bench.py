@@ -245,24 +245,41 @@ def main_joint(args):
 
 
 def main_copy(args):
-    """Device copy of 256 MiB (torch clone), the calibration run for the
-    FETCH_SIZE / WRITE_SIZE counters: a copy of known size reads and writes 256 MiB."""
+    """Device-memory bandwidth of plain copies (torch clone, read + write) and
+    reads (torch sum) of 64 MiB to 2 GiB: the ceilings the sweep kernels'
+    measured traffic is read against, and the calibration run for the
+    FETCH_SIZE / WRITE_SIZE counters (a copy of known size).  `value` is the
+    256 MiB copy."""
     import torch
 
-    n = 64 * 1024 * 1024
-    x = torch.ones(n, device="cuda:0")
-    for _ in range(max(args.warmup, 1)):
-        x.clone()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        x.clone()
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    line = {"metric": "device copy bandwidth (read + write)", "value": 2 * 4 * n / dt / 1e9, "unit": "GB/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+    def rate(fn, nbytes):
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        return nbytes / dt / 1e9, dt
+
+    table = {}
+    value, ms = None, None
+    for mib in (64, 256, 1024, 2048):
+        x = torch.ones(mib * 256 * 1024, device="cuda:0")
+        nb = mib * 1024 * 1024
+        cp, dt = rate(lambda: x.clone(), 2 * nb)
+        rd, _ = rate(lambda: x.sum(), nb)
+        table["%d MiB" % mib] = {"copy_GBps": round(cp, 1), "read_GBps": round(rd, 1)}
+        if mib == 256:
+            value, ms = cp, dt * 1e3
+        del x
+        torch.cuda.empty_cache()
+    line = {"metric": "device copy bandwidth (read + write)", "value": value, "unit": "GB/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "hbm_copy: 256 MiB torch clone (counter calibration)"}}
+            "config": {"workload": "hbm_copy: 256 MiB torch clone (counter calibration); copies and reads of "
+                                   "64 MiB to 2 GiB", "sizes": table}}
     print(json.dumps(line), flush=True)
 
 

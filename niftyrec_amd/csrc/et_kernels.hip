@@ -1558,7 +1558,11 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
     const bool coop = sweep_lds_bytes(R, TS, nthr, true, 1) <= lds_limit();
     const bool uses = mode == ET_MODE_PROJECT || (mode == ET_MODE_BPU && has_mu == 1);
     const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses, single ? 1 : 2);
+    // only the slab heights slab_rows() selects are instantiated per mode
     if (mode == ET_MODE_PROJECT) {
+        if constexpr (TS != 16) {
+            return (int)hipErrorInvalidValue;
+        } else {
         dim3 g(slabs, a.V * a.D);
 #define ET_LP(mu, cp, at)                                                                              \
     {                                                                                                  \
@@ -1581,7 +1585,11 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
             if (coop) ET_LP(false, true, 0) else ET_LP(false, false, 0)
         }
 #undef ET_LP
+        }
     } else if (mode == ET_MODE_BPU) {
+        if constexpr (TS < 8) {
+            return (int)hipErrorInvalidValue;
+        } else {
         dim3 g(slabs, a.V * a.D);
 #define ET_LB(mu, cp, at)                                                                              \
     {                                                                                                  \
@@ -1602,6 +1610,7 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
             ET_LB(false, false, false)
         }
 #undef ET_LB
+        }
     } else if (grp) {
         return (int)hipErrorInvalidValue;
     } else if (mode == ET_MODE_GRAD_REF) {
