@@ -293,7 +293,21 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
     p->slabs = (ny + ET_T - 1) / ET_T;
     p->xp = nx + 2;
     p->zp = nz + 2;
-    p->yp = ET_GY + p->slabs * ET_T + ET_GY;
+    // Guard rows of the YF layout: the sweeps gather the rows y0 - R4 ..
+    // y0 + TS + R4 - 1 of a column (R4 = R rounded up to 4).  With gy = R4 and
+    // a row stride of whole 16-float (64-byte) segments, the window of every
+    // 16-row slab starts on a segment boundary: a 32-row window touches two
+    // segments instead of three.  Plans off the fast path keep 16 rows.
+    if (p->fast) {
+        const int r4 = (p->R + 3) & ~3;
+        int tail = r4;
+        while ((r4 + tail) % 16) tail++;
+        p->gy = r4;
+        p->yp = r4 + p->slabs * ET_T + tail;
+    } else {
+        p->gy = ET_GY;
+        p->yp = ET_GY + p->slabs * ET_T + ET_GY;
+    }
     p->yu = p->slabs * ET_T;
     p->yf_elems = (size_t)p->yp * p->xp * p->zp;
     p->ustride = (size_t)p->yu * p->xp * p->zp;

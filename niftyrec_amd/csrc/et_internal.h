@@ -20,7 +20,7 @@
 #include <stdint.h>
 
 #define ET_T 16      // axial rows per slab in the sweep and pull kernels
-#define ET_GY 16     // guard rows of the YF layout (>= roundup(R, 4))
+#define ET_GY 16     // guard rows of the YF layout off the fast path (fast path: roundup(R, 4))
 #define ET_RMAX 10   // largest PSF radius handled inside the sweep kernels (K <= 21)
 #define ET_VIEW_REC 40  // floats per view record (see EtViewRec)
 #define ET_MAX_NX 512   // widest volume: sweep blocks hold one thread per rotated column (512 threads)
