@@ -327,9 +327,10 @@ def main():
     n_split = min(args.steps, 5)
     cat = max((k for k in split if split[k][0] > 0), key=lambda k: split[k][1])
 
-    # timed region: HIP events (on the plan's stream) only around the dominant kernel's launches
+    # timed region: HIP events (on the plan's stream) around every (S+1)-th launch
+    # of the dominant kernel: one per step, rotating over the subsets
     X.fill_(1.0)
-    plan.profile(True, categories=[cat])
+    plan.profile(True, categories=[cat], every=S + 1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -373,7 +374,8 @@ def main():
                      "depth_chunks": dchunks[cat == "bp_rotated_frame"]},
         "kernel_ms_per_step": {k: v[1] / n_split for k, v in split.items() if v[0]},
         "kernel_split_note": "per-kernel times from an untimed pass with every launch timed; the timed "
-                             "region times only the dominant kernel",
+                             "region times every %d-th launch of the dominant kernel (%d launches timed)"
+                             % (S + 1, launches),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, S)

@@ -228,6 +228,10 @@ int et_plan_get_tuning(const et_plan *plan, int *depth_chunks_project, int *dept
  * reset the accumulated figures. */
 int et_plan_profile(et_plan *plan, int enable);
 int et_plan_profile_select(et_plan *plan, unsigned int categories);
+/* Time only every `every`-th launch of each selected category (1 = all; the
+ * first launch after et_plan_profile/_select is timed): a caller measuring a
+ * kernel's mean launch time over a long run puts fewer events in the stream. */
+int et_plan_profile_sample(et_plan *plan, int every);
 int et_plan_profile_read(et_plan *plan, int category, int *launches, double *total_ms,
                          double *bytes);
 #define ET_PROF_PROJECT 0

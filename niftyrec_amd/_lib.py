@@ -68,6 +68,7 @@ SIGNATURES = {
     "et_plan_set_tuning": (c_int, [c_void_p, c_int, c_int]),
     "et_plan_get_tuning": (c_int, [c_void_p, P(c_int), P(c_int)]),
     "et_plan_profile_select": (c_int, [c_void_p, c_uint]),
+    "et_plan_profile_sample": (c_int, [c_void_p, c_int]),
     "et_plan_profile": (c_int, [c_void_p, c_int]),
     "et_plan_profile_read": (c_int, [c_void_p, c_int, ip, P(c_double), P(c_double)]),
     "et_library_version": (c_char_p, []),

@@ -283,9 +283,11 @@ class Plan:
         _lib.check(self._L.et_plan_get_tuning(self._h, ctypes.byref(dp), ctypes.byref(db)), "et_plan_get_tuning")
         return dp.value, db.value
 
-    def profile(self, enable=True, categories=None):
+    def profile(self, enable=True, categories=None, every=1):
         """Time the plan's launches with HIP events: every category, or only the
-        named ones (names of _lib.PROF_NAMES); resets the figures."""
+        named ones (names of _lib.PROF_NAMES), and of those every `every`-th
+        launch; resets the figures."""
+        _lib.check(self._L.et_plan_profile_sample(self._h, int(every)), "et_plan_profile_sample")
         if categories is None:
             _lib.check(self._L.et_plan_profile(self._h, int(bool(enable))), "et_plan_profile")
             return

@@ -238,6 +238,8 @@ struct et_plan {
     int tune_D = 0, tune_vb = 0;
     // profiling
     unsigned prof = 0;              // bit c set: launches of profiling category c are timed
+    int prof_every = 1;             // of those, every prof_every-th launch (counted in prof_seen)
+    long long prof_seen[ET_PROF_COUNT] = {0};
     std::vector<hipEvent_t> ev_pool;  // events reused by the profiling scopes
     struct Ev {
         int cat;
@@ -373,6 +375,7 @@ struct ProfScope {
     ProfScope(et_plan *p_, int cat_, double bytes_) : p(p_), cat(cat_), bytes(bytes_)
     {
         if (!((p->prof >> cat) & 1u)) return;
+        if (p->prof_seen[cat]++ % p->prof_every != 0) return;
         a = pool_event(p);
         b = pool_event(p);
         if (!a || !b) {
@@ -2087,7 +2090,16 @@ int et_plan_profile_select(et_plan *plan, unsigned int categories)
         plan->prof_ms[i] = 0;
         plan->prof_bytes[i] = 0;
         plan->prof_n[i] = 0;
+        plan->prof_seen[i] = 0;
     }
+    return niftyrec_success;
+}
+
+int et_plan_profile_sample(et_plan *plan, int every)
+{
+    if (!plan || every < 1) return niftyrec_error_parameters;
+    plan->prof_every = every;
+    for (int i = 0; i < ET_PROF_COUNT; i++) plan->prof_seen[i] = 0;
     return niftyrec_success;
 }
 

@@ -169,6 +169,24 @@ def test_plan_osem_fixed_depth_chunks():
         assert e <= 1e-5
 
 
+def test_plan_profile_sampling():
+    """et_plan_profile_select + et_plan_profile_sample: only the selected
+    category is timed, and of its launches every `every`-th one, starting with
+    the first."""
+    act, mu, cams, psf = _case(seed=15, V=8)
+    plan = _plan(act.shape, cams, psf)
+    y = _t(orc.project(act, mu, cams, psf))
+    X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
+    plan.osem(X, y, _t(mu), subsets=4, iterations=1)     # caches built outside the count
+    for every, want in ((1, 12), (3, 4), (5, 3)):
+        plan.profile(True, categories=["project"], every=every)
+        plan.osem(X, y, _t(mu), subsets=4, iterations=3)  # 12 projection launches
+        r = plan.profile_read()
+        plan.profile(False)
+        assert r["project"][0] == want and r["project"][1] > 0
+        assert r["bp_rotated_frame"][0] == 0 and r["bp_pull"][0] == 0
+
+
 @pytest.mark.parametrize("shape, V, K", [((96, 20, 96), 12, 7), ((160, 9, 160), 8, 3)])
 def test_plan_osem_wide_volumes(shape, V, K):
     """OSEM through the plan at widths that leave the last wave of a block
