@@ -477,6 +477,16 @@ __device__ __forceinline__ int2 xcd_order(const Blk &b, int S, int N, int fast0,
     return make_int2(seq % S, (seq / S) * 8 + (b.lin & 7));
 }
 
+// Contiguous form: XCD k runs the slow indices [k N/8, (k+1) N/8) in order, each
+// with its S fast blocks back to back, so neighbouring slow indices (which share
+// cache lines) meet in one L2.
+__device__ __forceinline__ int2 xcd_order_contig(const Blk &b, int S, int N, int fast0, int slow0)
+{
+    if ((N & 7) != 0) return make_int2(fast0, slow0);
+    const int seq = b.lin >> 3;
+    return make_int2(seq % S, (b.lin & 7) * (N >> 3) + seq / S);
+}
+
 template <int TS, int R>
 __device__ __forceinline__ SweepCtx sweep_ctx(const EtSweepArgs &a, bool chunked, const Blk &b)
 {
@@ -1020,8 +1030,10 @@ __device__ __forceinline__ void pull_body(const EtPullArgs &a, const Blk &b)
 {
     constexpr int SCR = ET_COOP_TAB + 64 * (T + 4);
     __shared__ float scr[ADJ ? 4 : 4 * SCR];
-    // fast index: axial slab (neighbouring slabs share cache lines of u)
-    const int2 fs = xcd_order(b, b.gy, b.gx, b.y, b.x);
+    // fast index: axial slab (neighbouring slabs share cache lines of u); slow:
+    // column blocks, contiguous per XCD (neighbouring blocks share the
+    // bilinear neighbours of their samples)
+    const int2 fs = xcd_order_contig(b, b.gy, b.gx, b.y, b.x);
     const int cbase = fs.y * blockDim.x;
     const int c0 = cbase + threadIdx.x;
     const bool cin = c0 < a.nx * a.nz;
