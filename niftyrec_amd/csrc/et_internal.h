@@ -20,7 +20,8 @@
 #include <stdint.h>
 
 #define ET_T 16      // axial rows per slab in the sweep and pull kernels
-#define ET_GY 16     // guard rows of the YF layout off the fast path (fast path: roundup(R, 4))
+#define ET_GY 16     // guard rows of mu in the YF layout (and of every volume off the fast path)
+#define ET_YF_SLACK 16  // zero floats after the last column of a packed mu (its halo windows' tail)
 #define ET_RMAX 10   // largest PSF radius handled inside the sweep kernels (K <= 21)
 #define ET_VIEW_REC 40  // floats per view record (see EtViewRec)
 #define ET_MAX_NX 512   // widest volume: sweep blocks hold one thread per rotated column (512 threads)
@@ -47,7 +48,8 @@ enum EtSource {
 
 struct EtSweepArgs {
     int nx, ny, nz;
-    int yp, xp, gy;          // YF layout of act/mu
+    int yp, xp, gy;          // YF layout of act (row stride, columns, guard rows)
+    int gmu;                 // guard rows of mu in the same layout
     int yu, ustride;         // U layout of the output volumes (per view stride)
     int V, D, zlen;          // views in launch, depth chunks, z' per chunk
     int slabs;               // ceil(ny / ET_T)

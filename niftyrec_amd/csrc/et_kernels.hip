@@ -586,9 +586,9 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             };
             if constexpr (COOP) {
                 float dummy[TS];
-                coop_rows<TS, TS, 0, OPM>(a.mu + (a.gy + c.y0), c.dx, c.dz, s, tab, stage, cum, dummy);
+                coop_rows<TS, TS, 0, OPM>(a.mu + (a.gmu + c.y0), c.dx, c.dz, s, tab, stage, cum, dummy);
             } else {
-                bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s,
+                bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s,
                                         [&](int i, float m) { A[i] += m; });
             }
         }
@@ -710,10 +710,10 @@ __device__ __forceinline__ void bpu_body(const EtSweepArgs &a, const Blk &b)
             };
             if constexpr (COOP) {
                 float rows[NA];
-                coop_rows<NA, LA, SA, OPA>(a.mu + (a.gy + c.y0 - R4), c.dx, c.dz, s, tab, stage, wfun, rows);
+                coop_rows<NA, LA, SA, OPA>(a.mu + (a.gmu + c.y0 - R4), c.dx, c.dz, s, tab, stage, wfun, rows);
                 column_pass<TS, R>(rows, kz + K, yc);
             } else {
-                resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K, wfun,
+                resample_column<TS, R, NA, LA, SA>(a.mu + s.off + (a.gmu + c.y0 - R4), c.dx, c.dz, s, kz + K, wfun,
                                                    yc);
             }
         } else if constexpr (R == 0) {
@@ -810,7 +810,7 @@ __global__ void __launch_bounds__(512) k_grad_ref(EtSweepArgs a)
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         // mu rows first: tail needs A(z') before this plane's mu joins the suffix
         float Az[NA];
-        bilinear_map<NA, LA, SA>(a.mu + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, [&](int j, float m) {
+        bilinear_map<NA, LA, SA>(a.mu + s.off + (a.gmu + c.y0 - R4), c.dx, c.dz, s, [&](int j, float m) {
             Az[j] = At[j] - suf[j];
             suf[j] += m;
         });
@@ -874,7 +874,7 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
             }
         }
         float out[TS];
-        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, [&](int i, float m) {
+        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s, [&](int i, float m) {
             const float Az = At[i] - suf[i];
             suf[i] += m;
             tail[i] = fmaf(h[i], exp_neg(Az), tail[i]);
@@ -895,7 +895,7 @@ __global__ void __launch_bounds__(512) k_musum(EtSweepArgs a)
     for (int i = 0; i < TS; i++) acc[i] = 0.f;
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
-        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gy + c.y0), c.dx, c.dz, s, [&](int i, float m) { acc[i] += m; });
+        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s, [&](int i, float m) { acc[i] += m; });
     }
     float *out = a.part + ((size_t)c.v * a.D + c.d) * c.P;
 #pragma unroll
