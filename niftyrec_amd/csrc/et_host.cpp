@@ -929,7 +929,8 @@ float *osem_att(et_plan *p, int v0)
 // Depth chunking of the projection (mode ET_MODE_PROJECT) or rotated-frame
 // backprojection (ET_MODE_BPU) sweep for nv views, timed on the plan's stream:
 // candidates 4..32 chunks (at least 4 planes each), three launches each after
-// one warm-up, fastest average kept.  More chunks mean more, shorter blocks
+// one warm-up, in three interleaved rounds; the candidate with the fastest
+// round is kept.  More chunks mean more, shorter blocks
 // (better balance over the CUs) at the cost of more partial sums.  Inputs are
 // the plan's own work buffers; the values computed are discarded.
 int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const float *mu_yf)
@@ -944,42 +945,44 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
     const unsigned prof = p->prof;
     p->prof = 0;
     const float *act = mu_yf ? mu_yf : p->act_yf.ensure(p->yf_elems, true, p->st);
-    int best = cand[0];
-    float best_ms = 1e30f;
-    for (int D : cand) {
-        const float *as = nullptr;  // both sweeps read the factors: no attenuation prefix
-        float *parts = p->part.ensure((size_t)D * nv * p->P, false);
-        p->u.ensure((size_t)nv * p->ustride, true, p->st);
-        // as in em_subset_update: both sweeps read the attenuation factors
-        float *att = mu_yf ? p->att.ensure(att_elems(p, nv), true, p->st) : nullptr;
-        float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
-        auto run = [&]() {
-            if (mode == ET_MODE_PROJECT) {
-                // with the ratio kernel, whose reads grow with the number of partial sums
-                project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts, att, true);
-                check_launch(et_launch_ratio_parts(parts, parts, D, (size_t)nv * p->P, 1e-6f, ratio,
-                                                   (et_stream_t)p->st));
-            } else {
-                bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
-                         att);
-            }
-        };
-        run();
-        check_hip(hipEventRecord(e0, p->st), niftyrec_error_kernel);
-        for (int r = 0; r < 3; r++) run();
-        check_hip(hipEventRecord(e1, p->st), niftyrec_error_kernel);
-        check_hip(hipEventSynchronize(e1), niftyrec_error_kernel);
-        float ms = 0.f;
-        check_hip(hipEventElapsedTime(&ms, e0, e1), niftyrec_error_kernel);
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = D;
+    // candidates interleaved over rounds; each keeps its fastest round, so a
+    // transient slowdown of one round does not decide the choice
+    constexpr int kRounds = 3;
+    std::vector<float> best_of(cand.size(), 1e30f);
+    for (int round = 0; round < kRounds; round++) {
+        for (size_t ci = 0; ci < cand.size(); ci++) {
+            const int D = cand[ci];
+            const float *as = nullptr;  // both sweeps read the factors: no attenuation prefix
+            float *parts = p->part.ensure((size_t)D * nv * p->P, false);
+            p->u.ensure((size_t)nv * p->ustride, true, p->st);
+            // as in em_subset_update: both sweeps read the attenuation factors
+            float *att = mu_yf ? p->att.ensure(att_elems(p, nv), true, p->st) : nullptr;
+            float *ratio = p->ratio.ensure((size_t)nv * p->P, false);
+            auto run = [&]() {
+                if (mode == ET_MODE_PROJECT) {
+                    // with the ratio kernel, whose reads grow with the number of partial sums
+                    project_fast(p, recs, nv, act, mu_yf, nullptr, 1, D, as, parts, att, true);
+                    check_launch(et_launch_ratio_parts(parts, parts, D, (size_t)nv * p->P, 1e-6f, ratio,
+                                                       (et_stream_t)p->st));
+                } else {
+                    bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
+                             att);
+                }
+            };
+            run();
+            check_hip(hipEventRecord(e0, p->st), niftyrec_error_kernel);
+            for (int r = 0; r < 3; r++) run();
+            check_hip(hipEventRecord(e1, p->st), niftyrec_error_kernel);
+            check_hip(hipEventSynchronize(e1), niftyrec_error_kernel);
+            float ms = 0.f;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), niftyrec_error_kernel);
+            best_of[ci] = std::min(best_of[ci], ms);
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     p->prof = prof;
-    return best;
+    return cand[std::min_element(best_of.begin(), best_of.end()) - best_of.begin()];
 }
 
 // Fast-path OSEM in three stages: osem_prepare (attenuation packed, subset
