@@ -933,6 +933,21 @@ float *osem_att(et_plan *p, int v0)
 // round is kept.  More chunks mean more, shorter blocks
 // (better balance over the CUs) at the cost of more partial sums.  Inputs are
 // the plan's own work buffers; the values computed are discarded.
+// Depth chunking without timing, for one-shot calls: the smallest power of two
+// (up to 32, at least 4 planes per chunk) whose launch holds about four waves
+// per SIMD of the device.
+int launch_depth_chunks(const et_plan *p, int nv)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const long waves_per_chunk = (long)p->slabs * nv * (nthreads_for(p->nx) / 64);
+    int D = 1;
+    while (D < 32 && p->nz / (2 * D) >= 4 && waves_per_chunk * D < 16L * cus) D *= 2;
+    return D;
+}
+
 int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const float *mu_yf)
 {
     std::vector<int> cand;
@@ -1273,6 +1288,8 @@ int em_step_host(float *activity, int size_x, int size_y, const float *sino, int
         plan.st = hc.st;
         rc = build_plan(&plan, nx, ny, nz, cams, psf, psf ? psf_h : 0, bg, bga);
         if (rc) return rc;
+        // one update: the chunking comes from the launch size, not from timing
+        plan.tune_D = launch_depth_chunks(&plan, V);
         const size_t n = (size_t)nx * ny * nz;
         float *d_x = hc.upload(activity, n);
         const float *d_meas = hc.upload(sino, (size_t)nx * ny * V);
