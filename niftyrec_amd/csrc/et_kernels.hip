@@ -229,7 +229,10 @@ __host__ __device__ constexpr int coop_owners(int LA, int floats)
 // consecutive 16-byte chunks of that owner's four corner runs (coalesced),
 // combine them with the owner's weights and store the owner's bilinear rows to
 // `stage` (OPP owners at a time, row stride LA + 4), from which the owners read
-// their runs back.  `stage` and `tab` belong to the calling wave alone.
+// their runs back.  `stage` and `tab` belong to the calling wave alone.  A
+// round's records are read after the previous round's stage write (they share
+// LDS), which keeps one round of loads in flight: issuing every round's loads
+// together was slower for both the projector and the pull.
 template <int NA, int LA, int SA, int OPP, class F>
 __device__ __forceinline__ void coop_rows(const float *__restrict__ base, int dx, int dz, const Samp &s,
                                           float *tab, float *stage, F f, float *out)
@@ -252,13 +255,20 @@ __device__ __forceinline__ void coop_rows(const float *__restrict__ base, int dx
     float run[LA];
 #pragma unroll
     for (int ph = 0; ph < PH; ph++) {
+        int off[RPP];
+        float w[RPP][4];
 #pragma unroll
         for (int r = 0; r < RPP; r++) {
             const int tt = r * RPR + tl;
             const int t = ph * OPP + tt;
             if (serve && tt < OPP && t < 64) {
-                const float *q = base + __float_as_int(tab[t]) + 4 * k;
-                const float w00 = tab[64 + t], w01 = tab[128 + t], w10 = tab[192 + t], w11 = tab[256 + t];
+                off[r] = __float_as_int(tab[t]);
+                w[r][0] = tab[64 + t];
+                w[r][1] = tab[128 + t];
+                w[r][2] = tab[192 + t];
+                w[r][3] = tab[256 + t];
+                const float *q = base + off[r] + 4 * k;
+                const float w00 = w[r][0], w01 = w[r][1], w10 = w[r][2], w11 = w[r][3];
                 const float4 a = gload4(q);
                 const float4 b = gload4(q + dx);
                 const float4 c = gload4(q + dz);
