@@ -367,6 +367,24 @@ __device__ __forceinline__ Samp fast_sample(const FastView &fv, float fxp, int z
     return make_samp(px, pz, nx, nz, dx, dz);
 }
 
+// Sum over d < D of part[d * stride + i], added in order of d.  The loads go out
+// eight at a time: a loop that adds each value as it arrives waits out one
+// memory latency per term.
+__device__ __forceinline__ float sum_parts(const float *part, int D, size_t stride, size_t i)
+{
+    float p = 0.f;
+    int d = 0;
+    for (; d + 8 <= D; d += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = gload(part + (size_t)(d + k) * stride + i);
+#pragma unroll
+        for (int k = 0; k < 8; k++) p += v[k];
+    }
+    for (; d < D; d++) p += gload(part + (size_t)d * stride + i);
+    return p;
+}
+
 // Input projection value s(v, y, x) for the backprojection sweeps.
 __device__ __forceinline__ float load_source(const EtSweepArgs &a, int v, int y, int x)
 {
@@ -376,8 +394,7 @@ __device__ __forceinline__ float load_source(const EtSweepArgs &a, int v, int y,
     if (a.src == ET_SRC_SINO) return gload(a.sino + (size_t)v * P + r);
     // ET_SRC_RATIO: reference _et_array_interface.cpp:1098-1108 (projection
     // truncated at 0, r = p < eps ? y/eps : y/p).
-    float p = 0.f;
-    for (int d = 0; d < a.rD; d++) p += gload(a.rpart + ((size_t)d * a.V + v) * P + r);
+    float p = sum_parts(a.rpart, a.rD, (size_t)a.V * P, (size_t)v * P + r);
     if (p < 0.f) p = 0.f;
     const float m = gload(a.meas + (size_t)v * P + r);
     return (p < a.eps) ? m / a.eps : m / p;
@@ -1372,8 +1389,7 @@ __global__ void __launch_bounds__(256) k_reduce_parts(const float *__restrict__ 
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= VP) return;
-    float s = 0.f;
-    for (int d = 0; d < D; d++) s += part[(size_t)d * VP + i];
+    float s = sum_parts(part, D, VP, i);
     if (truncate && s < 0.f) s = 0.f;
     sino[i] = s;
 }
@@ -1491,8 +1507,7 @@ __global__ void __launch_bounds__(256) k_ratio_parts_group(const EtRatioArgs *__
     const EtRatioArgs a = args[j];
     const size_t i = (size_t)b.lin * blockDim.x + threadIdx.x;
     if (i >= (size_t)a.VP) return;
-    float p = 0.f;
-    for (int d = 0; d < a.D; d++) p += a.part[(size_t)d * a.VP + i];
+    float p = sum_parts(a.part, a.D, (size_t)a.VP, i);
     if (p < 0.f) p = 0.f;
     a.out[i] = (p < a.eps) ? a.meas[i] / a.eps : a.meas[i] / p;
 }
@@ -1502,8 +1517,7 @@ __global__ void __launch_bounds__(256) k_ratio_parts(const float *__restrict__ m
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= VP) return;
-    float p = 0.f;
-    for (int d = 0; d < D; d++) p += part[(size_t)d * VP + i];
+    float p = sum_parts(part, D, VP, i);
     if (p < 0.f) p = 0.f;
     out[i] = (p < eps) ? meas[i] / eps : meas[i] / p;
 }
