@@ -930,7 +930,8 @@ float *osem_att(et_plan *p, int v0)
 // backprojection (ET_MODE_BPU) sweep for nv views, timed on the plan's stream:
 // candidates 4..32 chunks (at least 4 planes each), three launches each after
 // one warm-up, in three interleaved rounds; the candidate with the fastest
-// round is kept.  More chunks mean more, shorter blocks
+// round is kept.  A backprojection candidate is timed behind the projection
+// (Dp chunks) and ratio of the same views, as in the subset update.  More chunks mean more, shorter blocks
 // (better balance over the CUs) at the cost of more partial sums.  Inputs are
 // the plan's own work buffers; the values computed are discarded.
 // Depth chunking without timing, for one-shot calls: the smallest power of two
@@ -948,7 +949,7 @@ int launch_depth_chunks(const et_plan *p, int nv)
     return D;
 }
 
-int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const float *mu_yf)
+int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const float *mu_yf, int Dp = 0)
 {
     std::vector<int> cand;
     for (int D = 4; D <= 32; D *= 2)
@@ -968,7 +969,7 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
         for (size_t ci = 0; ci < cand.size(); ci++) {
             const int D = cand[ci];
             const float *as = nullptr;  // both sweeps read the factors: no attenuation prefix
-            float *parts = p->part.ensure((size_t)D * nv * p->P, false);
+            float *parts = p->part.ensure((size_t)std::max(D, Dp) * nv * p->P, false);
             p->u.ensure((size_t)nv * p->ustride, true, p->st);
             // as in em_subset_update: both sweeps read the attenuation factors
             float *att = mu_yf ? p->att.ensure(att_elems(p, nv), true, p->st) : nullptr;
@@ -980,7 +981,12 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
                     check_launch(et_launch_ratio_parts(parts, parts, D, (size_t)nv * p->P, 1e-6f, ratio,
                                                        (et_stream_t)p->st));
                 } else {
-                    bpu_fast(p, recs, nv, mu_yf, ET_SRC_ONES, nullptr, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
+                    // behind the chosen projection and its ratio, reading that ratio, as in
+                    // the subset update (the cache state the backprojection meets there)
+                    project_fast(p, recs, nv, act, mu_yf, nullptr, 1, Dp, as, parts, att, true);
+                    check_launch(et_launch_ratio_parts(parts, parts, Dp, (size_t)nv * p->P, 1e-6f, ratio,
+                                                       (et_stream_t)p->st));
+                    bpu_fast(p, recs, nv, mu_yf, ET_SRC_SINO, ratio, nullptr, nullptr, 0, 0.f, D, as, false, p->u.p,
                              att);
                 }
             };
@@ -1040,7 +1046,7 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
         int Dp = D, Db = D;
         if (p->tune_D <= 0) {
             Dp = tune_depth_chunks(p, ET_MODE_PROJECT, p->osem_recs.p, start[1], mu_yf);
-            Db = tune_depth_chunks(p, ET_MODE_BPU, p->osem_recs.p, start[1], mu_yf);
+            Db = tune_depth_chunks(p, ET_MODE_BPU, p->osem_recs.p, start[1], mu_yf, Dp);
         }
         p->osem_Dp = Dp;
         p->osem_Db = Db;
