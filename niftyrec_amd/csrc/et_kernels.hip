@@ -1594,58 +1594,56 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
     const bool coop = sweep_lds_bytes(R, TS, nthr, true, 1) <= lds_limit();
     const bool uses = mode == ET_MODE_PROJECT || (mode == ET_MODE_BPU && has_mu == 1);
     const size_t lds = sweep_lds_bytes(R, TS, nthr, coop && uses, single ? 1 : 2);
+#define ET_LP(mu, cp, at)                                                                                  \
+    {                                                                                                      \
+        if (grp) {                                                                                         \
+            allow_lds(k_project_group<TS, R, mu, cp, at>, lds);                                            \
+            hipLaunchKernelGGL((k_project_group<TS, R, mu, cp, at>), dim3(grp->blocks), dim3(nthr), lds, st, \
+                               grp->args, grp->tab, grp->n);                                               \
+        } else {                                                                                           \
+            allow_lds(k_project<TS, R, mu, cp, at>, lds);                                                  \
+            hipLaunchKernelGGL((k_project<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);                 \
+        }                                                                                                  \
+    }
+#define ET_LB(mu, cp, at)                                                                                  \
+    {                                                                                                      \
+        if (grp) {                                                                                         \
+            allow_lds(k_bpu_group<TS, R, mu, cp, at>, lds);                                                \
+            hipLaunchKernelGGL((k_bpu_group<TS, R, mu, cp, at>), dim3(grp->blocks), dim3(nthr), lds, st,     \
+                               grp->args, grp->tab, grp->n);                                               \
+        } else {                                                                                           \
+            allow_lds(k_bpu<TS, R, mu, cp, at>, lds);                                                      \
+            hipLaunchKernelGGL((k_bpu<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);                     \
+        }                                                                                                  \
+    }
     // only the slab heights slab_rows() selects are instantiated per mode
     if (mode == ET_MODE_PROJECT) {
-        if constexpr (TS != 16) {
+        if constexpr (TS == 16) {
+            dim3 g(slabs, a.V * a.D);
+            if (has_mu == 3) {
+                if (coop) ET_LP(true, true, 2) else ET_LP(true, false, 2)
+            } else if (has_mu == 2) {
+                if (coop) ET_LP(true, true, 1) else ET_LP(true, false, 1)
+            } else if (has_mu) {
+                if (coop) ET_LP(true, true, 0) else ET_LP(true, false, 0)
+            } else {
+                if (coop) ET_LP(false, true, 0) else ET_LP(false, false, 0)
+            }
+        } else {
             return (int)hipErrorInvalidValue;
-        } else {
-        dim3 g(slabs, a.V * a.D);
-#define ET_LP(mu, cp, at)                                                                              \
-    {                                                                                                  \
-        if (grp) {                                                                                     \
-            allow_lds(k_project_group<TS, R, mu, cp, at>, lds);                                        \
-            hipLaunchKernelGGL((k_project_group<TS, R, mu, cp, at>), dim3(grp->blocks), dim3(nthr), lds, \
-                               st, grp->args, grp->tab, grp->n);                                       \
-        } else {                                                                                       \
-            allow_lds(k_project<TS, R, mu, cp, at>, lds);                                              \
-            hipLaunchKernelGGL((k_project<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);             \
-        }                                                                                              \
-    }
-        if (has_mu == 3) {
-            if (coop) ET_LP(true, true, 2) else ET_LP(true, false, 2)
-        } else if (has_mu == 2) {
-            if (coop) ET_LP(true, true, 1) else ET_LP(true, false, 1)
-        } else if (has_mu) {
-            if (coop) ET_LP(true, true, 0) else ET_LP(true, false, 0)
-        } else {
-            if (coop) ET_LP(false, true, 0) else ET_LP(false, false, 0)
-        }
-#undef ET_LP
         }
     } else if (mode == ET_MODE_BPU) {
-        if constexpr (TS < 8) {
+        if constexpr (TS >= 8) {
+            dim3 g(slabs, a.V * a.D);
+            if (has_mu == 3) {
+                ET_LB(true, false, true)
+            } else if (has_mu) {
+                if (coop) ET_LB(true, true, false) else ET_LB(true, false, false)
+            } else {
+                ET_LB(false, false, false)
+            }
+        } else {
             return (int)hipErrorInvalidValue;
-        } else {
-        dim3 g(slabs, a.V * a.D);
-#define ET_LB(mu, cp, at)                                                                              \
-    {                                                                                                  \
-        if (grp) {                                                                                     \
-            allow_lds(k_bpu_group<TS, R, mu, cp, at>, lds);                                            \
-            hipLaunchKernelGGL((k_bpu_group<TS, R, mu, cp, at>), dim3(grp->blocks), dim3(nthr), lds, st, \
-                               grp->args, grp->tab, grp->n);                                           \
-        } else {                                                                                       \
-            allow_lds(k_bpu<TS, R, mu, cp, at>, lds);                                                  \
-            hipLaunchKernelGGL((k_bpu<TS, R, mu, cp, at>), g, dim3(nthr), lds, st, a);                 \
-        }                                                                                              \
-    }
-        if (has_mu == 3) {
-            ET_LB(true, false, true)
-        } else if (has_mu) {
-            if (coop) ET_LB(true, true, false) else ET_LB(true, false, false)
-        } else {
-            ET_LB(false, false, false)
-        }
-#undef ET_LB
         }
     } else if (grp) {
         return (int)hipErrorInvalidValue;
@@ -1656,6 +1654,8 @@ static int launch_sweep_ts(int mode, int has_mu, const EtSweepArgs &a, hipStream
     } else {
         return (int)hipErrorInvalidValue;
     }
+#undef ET_LP
+#undef ET_LB
     return (int)hipGetLastError();
 }
 
