@@ -134,3 +134,22 @@ def test_no_device_is_a_status_not_an_exit():
     handle = ctypes.c_void_p()
     assert L.et_plan_create(ctypes.byref(handle), 8, 6, 8, np.zeros(4, np.float32).ctypes.data_as(_lib.fp), 4, 1,
                             None, 0, 0.0, 0.0) == 8
+
+
+def test_bench_finds_its_traffic_rows():
+    """bench.py reads the measured HBM traffic of the dominant kernel from the
+    committed rocprofv3 summary: the row of the subset launch grid (16-row
+    slabs x 15 views x depth chunks, 128 threads at 128^3) must be there for
+    the projector and the rotated-frame backprojection at 16 chunks (the
+    chunking the profiler runs fix), and the pull's row by dispatch count."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for cat in ("project", "bp_rotated_frame"):
+        grid = bench.sweep_grid(cat, 128, 128, 15, 16)
+        assert grid == 8 * 15 * 16 * 128
+        t = bench.measured_traffic(cat, grid)
+        assert t is not None and 1e8 < t < 1e9, (cat, t)
+    assert bench.measured_traffic("bp_pull") is not None
+    assert bench.sweep_grid("bp_pull", 128, 128, 15, 16) is None
