@@ -1006,18 +1006,13 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
     return cand[std::min_element(best_of.begin(), best_of.end()) - best_of.begin()];
 }
 
-// Fast-path OSEM in three stages: osem_prepare (attenuation packed, subset
-// caches built or reused, measured planes gathered in subset order, x packed
+// Fast-path OSEM in three stages: osem_prepare (subset caches built, with the
+// attenuation packed, or reused, measured planes gathered in subset order, x packed
 // into the YF layout), osem_subset (one subset update on the YF iterate) and
 // osem_finish (x unpacked).  The subset layout stays in the plan between them.
 void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int S, float eps)
 {
     if (S <= 0 || S > p->V || !p->fast) throw EtError{niftyrec_error_parameters};
-    const float *mu_yf = nullptr;
-    if (mu) {
-        pack(p, mu, p->mu_yf);
-        mu_yf = p->mu_yf.p;
-    }
     // subset s: views s, s+S, ...; records and measured planes reordered by subset
     std::vector<int> order, start(S + 1, 0);
     for (int s = 0; s < S; s++) {
@@ -1028,6 +1023,10 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
     const int maxsub = (p->V + S - 1) / S;
     const int D = choose_D(p, maxsub);
     const bool rebuild = p->osem_S != S || p->osem_D != D || p->osem_mu_key != mu || p->osem_eps != eps;
+    // mu in the YF layout is read only to build the caches (the subset loop
+    // reads the cached factors), so it is packed only for a rebuild
+    if (mu && rebuild) pack(p, mu, p->mu_yf);
+    const float *mu_yf = mu ? p->mu_yf.p : nullptr;
     if (rebuild) {
         std::vector<float> recs((size_t)p->V * ET_VIEW_REC);
         for (int i = 0; i < p->V; i++)
