@@ -23,28 +23,48 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + os.path.join(ROOT, "include"),
           "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
-SOURCES = [("et_kernels.hip", ["-ffp-contract=fast"]),
-           ("et_host.cpp", ["-x", "hip", "-ffp-contract=off"])]
+RMAX = 10  # largest PSF radius of the sweep kernels (ET_RMAX in et_internal.h)
+# (source, object name, extra flags); the sweep launchers are instantiated once
+# per PSF radius in their own translation unit so that hipcc runs in parallel
+SOURCES = ([("et_kernels.hip", "et_kernels", ["-ffp-contract=fast"])] +
+           [("et_sweep_r.hip", "et_sweep_r%d" % r, ["-ffp-contract=fast", "-DET_SWEEP_R=%d" % r])
+            for r in range(RMAX, -1, -1)] +
+           [("et_host.cpp", "et_host", ["-x", "hip", "-ffp-contract=off"])])
+
+
+def _jobs():
+    try:
+        n = int(os.environ.get("MAX_JOBS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(n or (os.cpu_count() or 1), 16))
 
 
 def build(verbose=True):
-    """Compile every HIP source and link the shared library (always rebuilds)."""
+    """Compile every HIP source (in parallel) and link the shared library
+    (always rebuilds)."""
+    from concurrent.futures import ThreadPoolExecutor
     with tempfile.TemporaryDirectory(prefix="et_build_") as tmp:
-        objs = []
-        for src, extra in SOURCES:
-            obj = os.path.join(tmp, os.path.splitext(src)[0] + ".o")
+        def compile_one(item):
+            src, name, extra = item
+            obj = os.path.join(tmp, name + ".o")
             cmd = [HIPCC] + COMMON + extra + ["-c", os.path.join(CSRC, src), "-o", obj]
-            if verbose:
-                print("[build]", " ".join(cmd), flush=True)
             r = subprocess.run(cmd, capture_output=True, text=True)
             # the host pass does not know the packed-fp32 feature: drop that warning
             err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
-            if r.returncode != 0:
-                sys.stderr.write(err + "\n")
-                raise RuntimeError("hipcc failed on " + src)
-            if err.strip() and verbose:
-                sys.stderr.write(err + "\n")
-            objs.append(obj)
+            return cmd, obj, r.returncode, err
+
+        objs = []
+        with ThreadPoolExecutor(_jobs()) as ex:
+            for cmd, obj, rc, err in ex.map(compile_one, SOURCES):
+                if verbose:
+                    print("[build]", " ".join(cmd), flush=True)
+                if rc != 0:
+                    sys.stderr.write(err + "\n")
+                    raise RuntimeError("hipcc failed: " + " ".join(cmd))
+                if err.strip() and verbose:
+                    sys.stderr.write(err + "\n")
+                objs.append(obj)
         out_tmp = LIB_PATH + ".tmp"
         cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", out_tmp] + objs
         if verbose:
