@@ -177,6 +177,28 @@ int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y
                   float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y,
                   float background, float background_attenuation, float epsilon,
                   int use_gpu);
+/* The drivers et_array_mlem and et_array_osem keep the reference's results
+ * (activity reset to 1, cameras from et_array_make_cameras, et_array_mlem:
+ * `iterations` MLEM updates; et_array_osem: `iterations` et_array_osem_step
+ * updates on subsets drawn from rand()) but run device-resident: data,
+ * attenuation and iterate are uploaded once, one plan serves every step and
+ * the MLEM sensitivity is computed once.  They return the first non-zero
+ * status (the reference adds the statuses up).
+ *
+ * Deterministic ordered subsets (no counterpart in the reference; the
+ * reconstruction-driver row of the reference's rec_spect tool,
+ * apps/command_line_tools/rec_spect.cpp:553-555, calls et_array_osem):
+ * `iterations` passes over n_subsets subsets, subset s holding cameras s,
+ * s + n_subsets, s + 2 n_subsets, ... in that order; each subset update is
+ * the reference MLEM update restricted to its cameras (the update of
+ * et_plan_osem).  Other arguments as et_array_osem. */
+int et_array_osem_ordered(float *activity_data, unsigned int size_x, unsigned int size_y,
+                          unsigned int n_subsets, float *sinogram_data, int n_cameras,
+                          float firstcamera, float lastcamera, unsigned int rotation_axis,
+                          int iterations, int use_attenuation, float *attenuation_data,
+                          int use_psf, float *psf_data, unsigned int psf_size_x,
+                          unsigned int psf_size_y, float background,
+                          float background_attenuation, float epsilon, int use_gpu);
 
 /* ---- Part 2: device-resident plans (no counterpart in the reference) ------- */
 /* A plan holds the geometry (cameras, centres, PSF, backgrounds) on the device
@@ -211,11 +233,22 @@ int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *
  * act is updated in place.  The sensitivity of every subset, the attenuation
  * factors e^{-A} of every view (4 bytes per voxel and view of device memory)
  * and the depth chunking (timed on the first call unless fixed with
- * et_plan_set_tuning) are cached in the plan, keyed on (subsets, mu pointer,
- * eps); a caller that rewrites mu in place calls et_plan_set_tuning to drop
- * the cache. */
+ * et_plan_set_tuning) are cached in the plan, keyed on (subsets, eps, the
+ * attenuation map).  By default the map is keyed on a fingerprint of its
+ * content, computed on the device at every call (one pass over mu and one
+ * stream synchronisation), so a map rewritten in place, or another map at the
+ * same address, rebuilds the caches. */
 int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, int subsets,
                  int iterations, float epsilon, void *stream);
+/* check != 0 (default): key the OSEM caches on the attenuation map's content
+ * fingerprint.  check == 0: key them on its address only; the caller then
+ * calls et_plan_invalidate whenever the map's content changes (this saves the
+ * fingerprint pass and the synchronisation per call). */
+int et_plan_set_mu_check(et_plan *plan, int check);
+/* Drops the cached OSEM data (rebuilt by the next et_plan_osem). */
+int et_plan_invalidate(et_plan *plan);
+/* HIP device the plan's buffers live on (the device selected when it was created). */
+int et_plan_get_device(const et_plan *plan, int *device);
 /* Tuning: depth chunks per ray (0 = automatic), views per backprojection batch (0 = automatic). */
 int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch);
 /* Depth chunks per ray of the cached OSEM data: projection and rotated-frame

@@ -54,6 +54,8 @@ SIGNATURES = {
                               c_uint, c_uint, c_float, c_float, c_float, c_int]),
     "et_array_osem": (c_int, [fp, c_uint, c_uint, c_uint, fp, c_int, c_float, c_float, c_uint, c_int, c_int, fp,
                               c_int, fp, c_uint, c_uint, c_float, c_float, c_float, c_int]),
+    "et_array_osem_ordered": (c_int, [fp, c_uint, c_uint, c_uint, fp, c_int, c_float, c_float, c_uint, c_int, c_int,
+                                      fp, c_int, fp, c_uint, c_uint, c_float, c_float, c_float, c_int]),
     "et_batch_create": (c_int, [P(c_void_p), P(c_void_p), c_int]),
     "et_batch_destroy": (c_int, [c_void_p]),
     "et_batch_osem": (c_int, [c_void_p, P(c_void_p), P(c_void_p), P(c_void_p), c_int, c_int, c_float, c_int,
@@ -66,6 +68,9 @@ SIGNATURES = {
                                              c_void_p]),
     "et_plan_osem": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_void_p]),
     "et_plan_set_tuning": (c_int, [c_void_p, c_int, c_int]),
+    "et_plan_set_mu_check": (c_int, [c_void_p, c_int]),
+    "et_plan_invalidate": (c_int, [c_void_p]),
+    "et_plan_get_device": (c_int, [c_void_p, P(c_int)]),
     "et_plan_get_tuning": (c_int, [c_void_p, P(c_int), P(c_int)]),
     "et_plan_profile_select": (c_int, [c_void_p, c_uint]),
     "et_plan_profile_sample": (c_int, [c_void_p, c_int]),

@@ -168,6 +168,55 @@ def osem_step(activity, sinogram, cameras, subset_order, psf=None, attenuation=N
     return act
 
 
+def _driver_args(sinogram, psf, attenuation):
+    sino, mu, psf = _f32(sinogram), _f32(attenuation), _f32(psf)
+    V, ny, nx = sino.shape
+    h = psf.shape[-1] if psf is not None else 0
+    act = np.zeros((nx, ny, nx), dtype=np.float32)
+    return sino, mu, psf, V, ny, nx, h, act
+
+
+def mlem(sinogram, first_deg, last_deg, iterations, psf=None, attenuation=None, rotation_axis=1, background=0.0,
+         background_attenuation=0.0, epsilon=1e-6):
+    """`iterations` MLEM updates from a uniform start (et_array_mlem): cameras
+    from first_deg to last_deg inclusive about `rotation_axis`; the data stay
+    in device memory for every iteration.  Returns the activity (nx, ny, nx)."""
+    L = _lib.load()
+    sino, mu, psf, V, ny, nx, h, act = _driver_args(sinogram, psf, attenuation)
+    st = L.et_array_mlem(_p(act), nx, ny, _p(sino), V, float(first_deg), float(last_deg), int(rotation_axis),
+                         int(iterations), int(mu is not None), _p(mu), int(psf is not None), _p(psf), h, h,
+                         float(background), float(background_attenuation), float(epsilon), 1)
+    _lib.check(st, "mlem")
+    return act
+
+
+def osem(sinogram, subset_order, first_deg, last_deg, iterations, psf=None, attenuation=None, rotation_axis=1,
+         background=0.0, background_attenuation=0.0, epsilon=1e-6):
+    """`iterations` reference OSEM steps from a uniform start (et_array_osem):
+    each on V // subset_order cameras drawn with replacement from glibc rand()."""
+    L = _lib.load()
+    sino, mu, psf, V, ny, nx, h, act = _driver_args(sinogram, psf, attenuation)
+    st = L.et_array_osem(_p(act), nx, ny, int(subset_order), _p(sino), V, float(first_deg), float(last_deg),
+                         int(rotation_axis), int(iterations), int(mu is not None), _p(mu), int(psf is not None),
+                         _p(psf), h, h, float(background), float(background_attenuation), float(epsilon), 1)
+    _lib.check(st, "osem")
+    return act
+
+
+def osem_ordered(sinogram, n_subsets, first_deg, last_deg, iterations, psf=None, attenuation=None,
+                 rotation_axis=1, background=0.0, background_attenuation=0.0, epsilon=1e-6):
+    """`iterations` passes over `n_subsets` ordered subsets (subset s: cameras
+    s, s + n_subsets, ...) from a uniform start (et_array_osem_ordered)."""
+    L = _lib.load()
+    sino, mu, psf, V, ny, nx, h, act = _driver_args(sinogram, psf, attenuation)
+    st = L.et_array_osem_ordered(_p(act), nx, ny, int(n_subsets), _p(sino), V, float(first_deg), float(last_deg),
+                                 int(rotation_axis), int(iterations), int(mu is not None), _p(mu),
+                                 int(psf is not None), _p(psf), h, h, float(background),
+                                 float(background_attenuation), float(epsilon), 1)
+    _lib.check(st, "osem_ordered")
+    return act
+
+
 def rotate(image, angles, centers, background=0.0, axis_order=0):
     """Rotation resampler (et_array_rotate), image (nz, ny, nx)."""
     L = _lib.load()
@@ -193,15 +242,18 @@ class Plan:
     """Device-resident projector for torch tensors (cuda / ROCm device memory).
 
     shape: (nz, ny, nx); cameras as for `project`; psf (nz, h, h) or None.
-    Tensors passed to the methods must be float32, contiguous, on the plan's
-    device; work is queued on torch's current stream."""
+    Tensors passed to the methods must be float32, contiguous, of the plan's
+    volume or sinogram shape and on the plan's device (checked before every
+    call: the kernels trust the sizes); work is queued on torch's current
+    stream."""
 
     def __init__(self, shape, cameras, psf=None, background=0.0, background_attenuation=0.0):
         L = _lib.load()
         nz, ny, nx = shape
-        self.shape = tuple(shape)
+        self.shape = tuple(int(v) for v in shape)
         cams, cols, V = _cams(cameras)
         self.n_views = V
+        self.sino_shape = (V, ny, nx)
         psf = _f32(psf)
         h = psf.shape[-1] if psf is not None else 0
         handle = ctypes.c_void_p()
@@ -210,6 +262,9 @@ class Plan:
         _lib.check(st, "et_plan_create")
         self._h = handle
         self._L = L
+        dev = ctypes.c_int()
+        _lib.check(L.et_plan_get_device(handle, ctypes.byref(dev)), "et_plan_get_device")
+        self.device_index = dev.value
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -232,6 +287,21 @@ class Plan:
             raise ValueError("expected a contiguous float32 GPU tensor")
         return ctypes.c_void_p(t.data_ptr())
 
+    def _arg(self, t, shape, name):
+        """Device pointer of `t` after checking dtype, layout, device and shape."""
+        if t is None:
+            return None
+        import torch
+        if not isinstance(t, torch.Tensor):
+            raise TypeError("%s must be a torch tensor" % name)
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError("%s has shape %s, expected %s" % (name, tuple(t.shape), tuple(shape)))
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError("%s: expected a contiguous float32 tensor" % name)
+        if not t.is_cuda or t.device.index != self.device_index:
+            raise ValueError("%s is on %s, the plan on cuda:%d" % (name, t.device, self.device_index))
+        return ctypes.c_void_p(t.data_ptr())
+
     @staticmethod
     def _stream():
         import torch
@@ -243,38 +313,63 @@ class Plan:
 
     # -- operators ------------------------------------------------------------
     def project(self, activity, attenuation=None, truncate=True, out=None):
+        if activity is None and attenuation is None:
+            raise ValueError("project needs an activity or an attenuation map")
         ref = activity if activity is not None else attenuation
-        nz, ny, nx = self.shape
-        out = self._new((self.n_views, ny, nx), ref) if out is None else out
-        st = self._L.et_plan_project(self._h, self._ptr(activity), self._ptr(attenuation), self._ptr(out),
-                                     int(bool(truncate)), self._stream())
+        a = self._arg(activity, self.shape, "activity")
+        m = self._arg(attenuation, self.shape, "attenuation")
+        out = self._new(self.sino_shape, ref) if out is None else out
+        o = self._arg(out, self.sino_shape, "out")
+        st = self._L.et_plan_project(self._h, a, m, o, int(bool(truncate)), self._stream())
         _lib.check(st, "et_plan_project")
         return out
 
     def backproject(self, sinogram, attenuation=None, mode=REFERENCE, truncate=True, out=None):
+        s = self._arg(sinogram, self.sino_shape, "sinogram")
+        m = self._arg(attenuation, self.shape, "attenuation")
         out = self._new(self.shape, sinogram) if out is None else out
-        st = self._L.et_plan_backproject(self._h, self._ptr(sinogram), self._ptr(attenuation), self._ptr(out),
-                                         int(mode), int(bool(truncate)), self._stream())
+        o = self._arg(out, self.shape, "out")
+        st = self._L.et_plan_backproject(self._h, s, m, o, int(mode), int(bool(truncate)), self._stream())
         _lib.check(st, "et_plan_backproject")
         return out
 
     def gradient_attenuation(self, sinogram, activity, attenuation, exact=False, truncate=True, out=None):
+        if activity is None or attenuation is None:
+            raise ValueError("gradient_attenuation needs the activity and the attenuation map")
+        s = self._arg(sinogram, self.sino_shape, "sinogram")
+        a = self._arg(activity, self.shape, "activity")
+        m = self._arg(attenuation, self.shape, "attenuation")
         out = self._new(self.shape, activity) if out is None else out
-        st = self._L.et_plan_gradient_attenuation(self._h, self._ptr(sinogram), self._ptr(activity),
-                                                  self._ptr(attenuation), self._ptr(out), int(bool(exact)),
-                                                  int(bool(truncate)), self._stream())
+        o = self._arg(out, self.shape, "out")
+        st = self._L.et_plan_gradient_attenuation(self._h, s, a, m, o, int(bool(exact)), int(bool(truncate)),
+                                                  self._stream())
         _lib.check(st, "et_plan_gradient_attenuation")
         return out
 
     def osem(self, activity, sinogram, attenuation=None, subsets=1, iterations=1, epsilon=1e-6):
-        """Ordered-subsets EM in place on `activity`."""
-        st = self._L.et_plan_osem(self._h, self._ptr(activity), self._ptr(sinogram), self._ptr(attenuation),
-                                  int(subsets), int(iterations), float(epsilon), self._stream())
+        """Ordered-subsets EM in place on `activity` (subset s: views s, s + S, ...).
+        The subset sensitivities and attenuation factors are cached in the
+        plan and rebuilt when the attenuation map's content changes (see
+        set_mu_check)."""
+        a = self._arg(activity, self.shape, "activity")
+        s = self._arg(sinogram, self.sino_shape, "sinogram")
+        m = self._arg(attenuation, self.shape, "attenuation")
+        st = self._L.et_plan_osem(self._h, a, s, m, int(subsets), int(iterations), float(epsilon), self._stream())
         _lib.check(st, "et_plan_osem")
         return activity
 
     def set_tuning(self, depth_chunks=0, view_batch=0):
         _lib.check(self._L.et_plan_set_tuning(self._h, int(depth_chunks), int(view_batch)), "et_plan_set_tuning")
+
+    def set_mu_check(self, check=True):
+        """True (default): osem keys its caches on a device fingerprint of the
+        attenuation map's content.  False: on its address only; call
+        invalidate() after changing the map in place."""
+        _lib.check(self._L.et_plan_set_mu_check(self._h, int(bool(check))), "et_plan_set_mu_check")
+
+    def invalidate(self):
+        """Drop the cached OSEM data."""
+        _lib.check(self._L.et_plan_invalidate(self._h), "et_plan_invalidate")
 
     def tuning(self):
         """(projection, backprojection) depth chunks of the cached OSEM data
@@ -340,10 +435,12 @@ class Batch:
         if len(activities) != n or len(sinograms) != n or (attenuations is not None and len(attenuations) != n):
             raise ValueError("one activity, sinogram (and attenuation) per plan")
         P = ctypes.c_void_p * n
-        acts = P(*[Plan._ptr(a).value for a in activities])
-        sinos = P(*[Plan._ptr(s).value for s in sinograms])
-        mus = None if attenuations is None else P(*[(Plan._ptr(m).value if m is not None else None)
-                                                    for m in attenuations])
+        acts = P(*[p._arg(a, p.shape, "activity %d" % i).value for i, (p, a) in enumerate(zip(self.plans, activities))])
+        sinos = P(*[p._arg(s, p.sino_shape, "sinogram %d" % i).value
+                    for i, (p, s) in enumerate(zip(self.plans, sinograms))])
+        mus = None if attenuations is None else P(*[(p._arg(m, p.shape, "attenuation %d" % i).value
+                                                     if m is not None else None)
+                                                    for i, (p, m) in enumerate(zip(self.plans, attenuations))])
         st = self._L.et_batch_osem(self._h, acts, sinos, mus, int(subsets), int(iterations), float(epsilon),
                                    int(bool(use_graph)), Plan._stream())
         _lib.check(st, "et_batch_osem")

@@ -5,6 +5,8 @@
 // includes it compiles its own copy.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <cstring>
 
 #include "et_internal.h"

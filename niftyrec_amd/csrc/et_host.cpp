@@ -17,6 +17,7 @@
 #include <new>
 #include <string>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <vector>
 
@@ -234,15 +235,22 @@ struct et_plan {
     size_t yf_elems = 0, ustride = 0, P = 0, nvox = 0;
     // workspaces
     DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol, att;
-    // OSEM caches
+    // OSEM caches (valid while osem_valid; keyed on the subset count, the
+    // chunking, eps and the attenuation map's key: its content fingerprint, or
+    // its address when mu_check is off)
     DevBuf x_yf, sens, osem_recs, osem_meas, osem_E, osem_order;
+    bool osem_valid = false;
     int osem_S = 0, osem_D = 0, osem_Dp = 0, osem_Db = 0;
     bool osem_mu_yf = false;
     std::vector<int> osem_start;    // subset s = views osem_start[s] .. osem_start[s+1]-1 (subset order)
     const float *osem_cur_mu = nullptr;
     float osem_cur_eps = 0.f;
-    const float *osem_mu_key = nullptr;
+    bool osem_has_mu = false;
+    unsigned long long osem_mu_key = 0;
     float osem_eps = 0.f;
+    int mu_check = 1;               // 1: fingerprint mu's content on every OSEM call
+    DevBuf fp_dev;                  // fingerprint (8 bytes)
+    unsigned long long *fp_host = nullptr;  // pinned copy of the fingerprint
     // tuning
     int tune_D = 0, tune_vb = 0;
     // profiling
@@ -262,6 +270,7 @@ struct et_plan {
 
     ~et_plan()
     {
+        if (fp_host) (void)hipHostFree(fp_host);
         for (auto &e : events) {
             (void)hipEventDestroy(e.a);
             (void)hipEventDestroy(e.b);
@@ -1010,7 +1019,35 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
 // attenuation packed, or reused, measured planes gathered in subset order, x packed
 // into the YF layout), osem_subset (one subset update on the YF iterate) and
 // osem_finish (x unpacked).  The subset layout stays in the plan between them.
-void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int S, float eps)
+// Key of the attenuation map for the OSEM caches: mu_key_launch queues its
+// content fingerprint (copied to pinned host memory) on the plan's stream;
+// mu_key_read returns it once the stream has reached that point (or mu's
+// address when the plan's mu_check is off, 0 without attenuation).
+void mu_key_launch(et_plan *p, const float *mu)
+{
+    if (!mu || !p->mu_check) return;
+    p->fp_dev.ensure(2, false);
+    if (!p->fp_host &&
+        hipHostMalloc((void **)&p->fp_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        p->fp_host = nullptr;
+        throw EtError{niftyrec_error_alloccpu};
+    }
+    ProfScope ps(p, ET_PROF_OTHER, 4.0 * p->nvox);
+    check_launch(et_launch_fingerprint(mu, p->nvox, reinterpret_cast<unsigned long long *>(p->fp_dev.p),
+                                       (et_stream_t)p->st));
+    check_hip(hipMemcpyAsync(p->fp_host, p->fp_dev.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->st),
+              niftyrec_error_transfergpu);
+}
+
+unsigned long long mu_key_read(const et_plan *p, const float *mu)
+{
+    if (!mu) return 0;
+    if (!p->mu_check) return (unsigned long long)(uintptr_t)mu;
+    return *p->fp_host;
+}
+
+void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int S, float eps,
+                  unsigned long long mu_key)
 {
     if (S <= 0 || S > p->V || !p->fast) throw EtError{niftyrec_error_parameters};
     // subset s: views s, s+S, ...; records and measured planes reordered by subset
@@ -1022,7 +1059,8 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
     start[S] = (int)order.size();
     const int maxsub = (p->V + S - 1) / S;
     const int D = choose_D(p, maxsub);
-    const bool rebuild = p->osem_S != S || p->osem_D != D || p->osem_mu_key != mu || p->osem_eps != eps;
+    const bool rebuild = !p->osem_valid || p->osem_S != S || p->osem_D != D || p->osem_has_mu != (mu != nullptr) ||
+                         p->osem_mu_key != mu_key || p->osem_eps != eps;
     // mu in the YF layout is read only to build the caches (the subset loop
     // reads the cached factors), so it is packed only for a rebuild
     if (mu && rebuild) pack(p, mu, p->mu_yf);
@@ -1041,7 +1079,7 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
                                  p->st),
                   niftyrec_error_transfergpu);
         // chunking: fixed by et_plan_set_tuning, else timed on the first subset
-        p->osem_S = 0;  // invalid until the rebuild completes
+        p->osem_valid = false;  // until the rebuild completes
         int Dp = D, Db = D;
         if (p->tune_D <= 0) {
             Dp = tune_depth_chunks(p, ET_MODE_PROJECT, p->osem_recs.p, start[1], mu_yf);
@@ -1073,8 +1111,10 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
         check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
         p->osem_S = S;
         p->osem_D = D;
-        p->osem_mu_key = mu;
+        p->osem_has_mu = mu != nullptr;
+        p->osem_mu_key = mu_key;
         p->osem_eps = eps;
+        p->osem_valid = true;
     }
     // measured data in subset order
     p->osem_meas.ensure((size_t)p->V * p->P, false);
@@ -1118,7 +1158,9 @@ void op_osem(et_plan *p, float *x, const float *meas, const float *mu, int S, in
         for (int it = 0; it < iters; it++) em_step_general(p, x, meas, mu, eps);
         return;
     }
-    osem_prepare(p, x, meas, mu, S, eps);
+    mu_key_launch(p, mu);
+    if (mu && p->mu_check) check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
+    osem_prepare(p, x, meas, mu, S, eps, mu_key_read(p, mu));
     for (int it = 0; it < iters; it++)
         for (int s = 0; s < S; s++) osem_subset(p, s);
     osem_finish(p, x);
@@ -1305,6 +1347,132 @@ int em_step_host(float *activity, int size_x, int size_y, const float *sino, int
     });
 }
 
+// Camera subset of et_array_osem_step, emission-lib/_et_array_interface.cpp:981-1009:
+// n_cameras / subset_order cameras drawn with replacement from glibc rand()
+// (two calls per camera, R = base + fine/scale in single precision); an index
+// equal to n_cameras (the reference's clamp) is clamped to n_cameras - 1.
+std::vector<int> random_subset(int n_cameras, int nsub)
+{
+    std::vector<int> idx(nsub);
+    const float scale = (float)(RAND_MAX + 1.);
+    for (int c = 0; c < nsub; c++) {
+        const float base = rand() / scale;
+        const float fine = rand() / scale;
+        const float Rr = base + fine / scale;
+        unsigned int i = (unsigned int)(n_cameras * Rr);
+        if (i >= (unsigned int)n_cameras) i = n_cameras - 1;
+        idx[c] = (int)i;
+    }
+    return idx;
+}
+
+// Cameras [3 x n] of the views idx of a [3 x V] camera array.
+std::vector<float> camera_subset(const float *cams3, int V, const std::vector<int> &idx)
+{
+    const int n = (int)idx.size();
+    std::vector<float> out((size_t)3 * n);
+    for (int c = 0; c < n; c++)
+        for (int axis = 0; axis < 3; axis++) out[c + axis * n] = cams3[idx[c] + axis * V];
+    return out;
+}
+
+enum EmDriver { EM_MLEM = 0, EM_OSEM_ORDERED = 1, EM_OSEM_RANDOM = 2 };
+
+// Device-resident EM drivers (the loops of emission-lib/_et_array_interface.cpp:940-960
+// and :1058-1076 without the per-step uploads): the data, attenuation and
+// iterate stay in device memory for every iteration, one plan's workspaces
+// and caches serve every step.
+//   EM_MLEM          `iterations` MLEM updates over all cameras (sensitivity once);
+//   EM_OSEM_ORDERED  `iterations` passes over `param` ordered subsets (subset s:
+//                    cameras s, s + S, ...), the update of et_plan_osem;
+//   EM_OSEM_RANDOM   `iterations` et_array_osem_step updates, each on
+//                    n_cameras / `param` cameras drawn with replacement from rand().
+// The activity is reset to 1 first, as the reference drivers do.
+int em_driver_host(int kind, float *activity, int size_x, int size_y, const float *sino, int V, const float *cams3,
+                   const float *attenuation, const float *psf, int psf_h, float bg, float bga, float eps, int param,
+                   int iterations)
+{
+    if (size_x <= 0 || size_y <= 0 || V <= 0 || iterations < 0) return niftyrec_error_parameters;
+    if (size_x > ET_MAX_NX) return niftyrec_error_parameters;
+    if (kind == EM_OSEM_ORDERED && (param <= 0 || param > V)) return niftyrec_error_parameters;
+    if (kind == EM_OSEM_RANDOM && (param <= 0 || V / param <= 0)) return niftyrec_error_parameters;
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        const int nx = size_x, ny = size_y, nz = size_x;
+        const int vs[3] = {nx, ny, nz};
+        const size_t n = (size_t)nx * ny * nz, P = (size_t)nx * ny;
+        HostCall hc;
+        hc.init();
+        et_plan plan;
+        plan.st = hc.st;
+        std::vector<float> ones(n, 1.0f);
+        float *d_x = hc.upload(ones.data(), n);
+        const float *d_meas = hc.upload(sino, P * V);
+        const float *d_mu = attenuation ? hc.upload(attenuation, n) : nullptr;
+        if (kind != EM_OSEM_RANDOM) {
+            Cameras cams;
+            int rc = expand_cameras(cams3, V, 3, vs, cams);
+            if (rc) return rc;
+            rc = build_plan(&plan, nx, ny, nz, cams, psf, psf ? psf_h : 0, bg, bga);
+            if (rc) return rc;
+            if (kind == EM_OSEM_ORDERED && param > 1 && !plan.fast) {
+                // off the fast path: one plan per subset, reference update on each
+                std::vector<std::unique_ptr<et_plan>> sub;
+                for (int s = 0; s < param; s++) {
+                    std::vector<int> idx;
+                    for (int v = s; v < V; v += param) idx.push_back(v);
+                    const std::vector<float> c = camera_subset(cams3, V, idx);
+                    Cameras cs;
+                    if ((rc = expand_cameras(c.data(), (int)idx.size(), 3, vs, cs))) return rc;
+                    sub.emplace_back(new et_plan());
+                    sub.back()->st = hc.st;
+                    if ((rc = build_plan(sub.back().get(), nx, ny, nz, cs, psf, psf ? psf_h : 0, bg, bga))) return rc;
+                }
+                float *d_msub = hc.alloc(P * ((V + param - 1) / param));
+                for (int it = 0; it < iterations; it++)
+                    for (int s = 0; s < param; s++) {
+                        int k = 0;
+                        for (int v = s; v < V; v += param, k++)
+                            check_hip(hipMemcpyAsync(d_msub + k * P, d_meas + (size_t)v * P, P * sizeof(float),
+                                                     hipMemcpyDeviceToDevice, hc.st),
+                                      niftyrec_error_transfergpu);
+                        em_step_general(sub[s].get(), d_x, d_msub, d_mu, eps);
+                    }
+            } else {
+                // chunking from the launch size, as the one-step entry points
+                plan.tune_D = launch_depth_chunks(&plan, kind == EM_MLEM ? V : (V + param - 1) / param);
+                op_osem(&plan, d_x, d_meas, d_mu, kind == EM_MLEM ? 1 : param, iterations, eps);
+            }
+        } else {
+            const int nsub = V / param;
+            float *d_msub = hc.alloc(P * nsub);
+            int *d_idx = reinterpret_cast<int *>(hc.alloc(nsub));
+            for (int it = 0; it < iterations; it++) {
+                const std::vector<int> idx = random_subset(V, nsub);
+                const std::vector<float> c = camera_subset(cams3, V, idx);
+                Cameras cs;
+                int rc = expand_cameras(c.data(), nsub, 3, vs, cs);
+                if (rc) return rc;
+                // the same plan object each step: its workspaces are reused, its
+                // OSEM caches rebuilt for the new cameras
+                rc = build_plan(&plan, nx, ny, nz, cs, psf, psf ? psf_h : 0, bg, bga);
+                if (rc) return rc;
+                plan.osem_valid = false;
+                plan.tune_D = launch_depth_chunks(&plan, nsub);
+                check_hip(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * nsub, hipMemcpyHostToDevice, hc.st),
+                          niftyrec_error_transfergpu);
+                check_launch(et_launch_gather_planes(d_msub, d_meas, d_idx, nsub, P, (et_stream_t)hc.st));
+                // the host copy of idx must outlive the transfer
+                check_hip(hipStreamSynchronize(hc.st), niftyrec_error_kernel);
+                op_osem(&plan, d_x, d_msub, d_mu, 1, 1, eps);
+            }
+        }
+        hc.download(activity, d_x, n);
+        return niftyrec_success;
+    });
+}
+
 thread_local char t_msg[2048];
 
 }  // namespace
@@ -1484,10 +1652,20 @@ void batch_issue(const et_batch *b, hipStream_t st)
 void op_batch_osem(et_batch *b, float *const *act, const float *const *sino, const float *const *mu, int S,
                    int iters, float eps, bool use_graph, hipStream_t st)
 {
+    // attenuation keys of every plan, read after one synchronisation
+    bool any_fp = false;
     for (size_t i = 0; i < b->plans.size(); i++) {
         et_plan *p = b->plans[i];
         p->st = st;
-        osem_prepare(p, act[i], sino[i], mu ? mu[i] : nullptr, S, eps);
+        const float *m = mu ? mu[i] : nullptr;
+        mu_key_launch(p, m);
+        any_fp = any_fp || (m && p->mu_check);
+    }
+    if (any_fp) check_hip(hipStreamSynchronize(st), niftyrec_error_kernel);
+    for (size_t i = 0; i < b->plans.size(); i++) {
+        et_plan *p = b->plans[i];
+        const float *m = mu ? mu[i] : nullptr;
+        osem_prepare(p, act[i], sino[i], m, S, eps, mu_key_read(p, m));
     }
     const std::vector<const void *> key = batch_key(b, S, eps);
     if (key != b->key || b->launches.empty()) {
@@ -1674,21 +1852,26 @@ int et_array_gradient_attenuation(float *sino, int *sino_size, float *activity, 
                                   float background_attenuation, int GPU, int truncate_negative_values)
 {
     (void)GPU;
-    // Validation, emission-lib/_et_array_interface.cpp:727-780 (the 2-D branch
-    // applies when the sinogram holds one plane).
+    // Validation, emission-lib/_et_array_interface.cpp:727-780.  The 2-D branch
+    // (sinogram [N, n_cameras, 1] with more than one camera) describes a volume
+    // of one axial row: the reference wraps the sinogram as
+    // [gradient_size[0], gradient_size[1], n_cameras] (:830-834), so the gradient
+    // must be [N, 1, N] and the N * n_cameras sinogram values are read as
+    // [N, 1, n_cameras].  Its PSF is the 3-D one, [h, h, N]; the [h, N]
+    // spelling of the 2-D check (:752) is refused.
     const bool no_psf = is_absent(psf_size) || psf == nullptr;
     const int V = cameras_size[0], cols = cameras_size[1];
     if (!(cols == 1 || cols == 3 || cols == 6) || V <= 0) return niftyrec_error_parameters;
-    if (sino_size[2] == 1) {
+    const bool two_d = sino_size[2] == 1 && V > 1;
+    if (two_d) {
         if (sino_size[1] != V) return niftyrec_error_parameters;
-        if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] != sino_size[0])) return niftyrec_error_parameters;
     } else {
         if (sino_size[2] != V) return niftyrec_error_parameters;
-        if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != sino_size[0]))
-            return niftyrec_error_parameters;
     }
+    if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != sino_size[0]))
+        return niftyrec_error_parameters;
     const int nx = gradient_size[0], ny = gradient_size[1], nz = gradient_size[2];
-    if (nx != sino_size[0] || ny != sino_size[1] || nz <= 0) return niftyrec_error_parameters;
+    if (nx != sino_size[0] || ny != (two_d ? 1 : sino_size[1]) || nz <= 0) return niftyrec_error_parameters;
     if (activity == nullptr || attenuation == nullptr) return niftyrec_error_parameters;
     if (!no_psf && (psf_size[0] != psf_size[1] || nz != nx)) return niftyrec_error_parameters;
     if (activity_size && (activity_size[0] != nx || activity_size[1] != ny || activity_size[2] != nz))
@@ -1983,17 +2166,11 @@ int et_array_osem_step(float *activity_data, int size_x, int size_y, unsigned in
     if (nsub <= 0) return niftyrec_error_parameters;
     if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
     const size_t P = (size_t)size_x * size_y;
-    std::vector<float> cams((size_t)3 * nsub), sino_sub(P * nsub);
-    const float scale = (float)(RAND_MAX + 1.);
-    for (int c = 0; c < nsub; c++) {
-        const float base = rand() / scale;
-        const float fine = rand() / scale;
-        const float Rr = base + fine / scale;
-        unsigned int idx = (unsigned int)(n_cameras * Rr);
-        if (idx >= (unsigned int)n_cameras) idx = n_cameras - 1;
-        for (int axis = 0; axis < 3; axis++) cams[c + axis * nsub] = cameras_data[idx + axis * n_cameras];
-        std::memcpy(&sino_sub[(size_t)c * P], sinogram_data + (size_t)idx * P, sizeof(float) * P);
-    }
+    const std::vector<int> idx = random_subset(n_cameras, nsub);
+    const std::vector<float> cams = camera_subset(cameras_data, n_cameras, idx);
+    std::vector<float> sino_sub(P * nsub);
+    for (int c = 0; c < nsub; c++)
+        std::memcpy(&sino_sub[(size_t)c * P], sinogram_data + (size_t)idx[c] * P, sizeof(float) * P);
     return em_step_host(activity_data, size_x, size_y, sino_sub.data(), nsub, cams.data(),
                         use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr,
                         (int)psf_size_x, background, background_attenuation, epsilon);
@@ -2004,14 +2181,14 @@ int et_array_mlem(float *activity_data, unsigned int size_x, unsigned int size_y
                   float *attenuation_data, int use_psf, float *psf_data, unsigned int psf_size_x,
                   unsigned int psf_size_y, float background, float background_attenuation, float epsilon, int use_gpu)
 {
-    for (size_t i = 0; i < (size_t)size_x * size_y * size_x; i++) activity_data[i] = 1.0f;
+    (void)use_gpu;
+    if (n_cameras <= 0 || rotation_axis > 2) return niftyrec_error_parameters;
+    if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
     std::vector<float> cams((size_t)3 * n_cameras);
-    int status = et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
-    for (int it = 0; it < iterations; it++)
-        status += et_array_mlem_step(activity_data, size_x, size_y, sinogram_data, n_cameras, cams.data(),
-                                     use_attenuation, attenuation_data, use_psf, psf_data, psf_size_x, psf_size_y,
-                                     background, background_attenuation, epsilon, use_gpu);
-    return status;
+    et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
+    return em_driver_host(EM_MLEM, activity_data, (int)size_x, (int)size_y, sinogram_data, n_cameras, cams.data(),
+                          use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr, (int)psf_size_x,
+                          background, background_attenuation, epsilon, 1, iterations);
 }
 
 int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y, unsigned int subset_order,
@@ -2020,14 +2197,31 @@ int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y
                   int use_psf, float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y, float background,
                   float background_attenuation, float epsilon, int use_gpu)
 {
-    for (size_t i = 0; i < (size_t)size_x * size_y * size_x; i++) activity_data[i] = 1.0f;
+    (void)use_gpu;
+    if (n_cameras <= 0 || rotation_axis > 2 || subset_order == 0) return niftyrec_error_parameters;
+    if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
     std::vector<float> cams((size_t)3 * n_cameras);
-    int status = et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
-    for (int it = 0; it < iterations; it++)
-        status += et_array_osem_step(activity_data, size_x, size_y, subset_order, sinogram_data, n_cameras,
-                                     cams.data(), use_attenuation, attenuation_data, use_psf, psf_data, psf_size_x,
-                                     psf_size_y, background, background_attenuation, epsilon, use_gpu);
-    return status;
+    et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
+    return em_driver_host(EM_OSEM_RANDOM, activity_data, (int)size_x, (int)size_y, sinogram_data, n_cameras,
+                          cams.data(), use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr,
+                          (int)psf_size_x, background, background_attenuation, epsilon, (int)subset_order,
+                          iterations);
+}
+
+int et_array_osem_ordered(float *activity_data, unsigned int size_x, unsigned int size_y, unsigned int n_subsets,
+                          float *sinogram_data, int n_cameras, float firstcamera, float lastcamera,
+                          unsigned int rotation_axis, int iterations, int use_attenuation, float *attenuation_data,
+                          int use_psf, float *psf_data, unsigned int psf_size_x, unsigned int psf_size_y,
+                          float background, float background_attenuation, float epsilon, int use_gpu)
+{
+    (void)use_gpu;
+    if (n_cameras <= 0 || rotation_axis > 2 || n_subsets == 0) return niftyrec_error_parameters;
+    if (use_psf && (psf_size_x % 2 != 1 || psf_size_x != psf_size_y)) return niftyrec_error_parameters;
+    std::vector<float> cams((size_t)3 * n_cameras);
+    et_array_make_cameras(cams.data(), firstcamera, lastcamera, n_cameras, rotation_axis);
+    return em_driver_host(EM_OSEM_ORDERED, activity_data, (int)size_x, (int)size_y, sinogram_data, n_cameras,
+                          cams.data(), use_attenuation ? attenuation_data : nullptr, use_psf ? psf_data : nullptr,
+                          (int)psf_size_x, background, background_attenuation, epsilon, (int)n_subsets, iterations);
 }
 
 // ---- device-resident plans ------------------------------------------------------
@@ -2108,14 +2302,36 @@ int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch_)
     if (!plan) return niftyrec_error_parameters;
     plan->tune_D = depth_chunks;
     plan->tune_vb = view_batch_;
-    plan->osem_S = 0;  // cached OSEM data depend on the chunking
+    plan->osem_valid = false;  // cached OSEM data depend on the chunking
+    return niftyrec_success;
+}
+
+int et_plan_get_device(const et_plan *plan, int *device)
+{
+    if (!plan || !device) return niftyrec_error_parameters;
+    *device = plan->device;
+    return niftyrec_success;
+}
+
+int et_plan_invalidate(et_plan *plan)
+{
+    if (!plan) return niftyrec_error_parameters;
+    plan->osem_valid = false;
+    return niftyrec_success;
+}
+
+int et_plan_set_mu_check(et_plan *plan, int check)
+{
+    if (!plan) return niftyrec_error_parameters;
+    plan->mu_check = check != 0;
+    plan->osem_valid = false;  // the key changes meaning
     return niftyrec_success;
 }
 
 int et_plan_get_tuning(const et_plan *plan, int *depth_chunks_project, int *depth_chunks_backproject)
 {
     if (!plan || !depth_chunks_project || !depth_chunks_backproject) return niftyrec_error_parameters;
-    const bool cached = plan->osem_S > 0;
+    const bool cached = plan->osem_valid;
     *depth_chunks_project = cached ? plan->osem_Dp : 0;
     *depth_chunks_backproject = cached ? plan->osem_Db : 0;
     return niftyrec_success;

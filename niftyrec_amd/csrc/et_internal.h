@@ -163,5 +163,8 @@ int et_launch_ratio_group(const EtRatioArgs *args, const void *tab, int n, int b
 // what lies beyond plane z, sino = the whole ray; act or mu may be null.
 int et_launch_partial_line(const float *act, const float *mu, const float *bgimg, float bg, int nx, int ny, int nz,
                            float *sino, float *part, et_stream_t stream);
+// Content fingerprint of n floats into *out (8 bytes of device memory): the
+// key under which the OSEM caches remember the attenuation map.
+int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).
 int et_sweep_slab_rows(int mode, int R);

@@ -614,6 +614,33 @@ __global__ void __launch_bounds__(256) k_ratio_parts(const float *__restrict__ m
     out[i] = (p < eps) ? meas[i] / eps : meas[i] / p;
 }
 
+// Content fingerprint of n floats: the sum modulo 2^64 of a 64-bit mix of
+// (bits, index) per element (order-independent, so the block partials are
+// combined with one atomic add each).  Any change of one element changes the
+// sum.  out must hold zero before the launch.
+__device__ __forceinline__ unsigned long long fp_mix(unsigned long long h)
+{
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+__global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict__ p, size_t n,
+                                                     unsigned long long *__restrict__ out)
+{
+    unsigned long long h = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        h += fp_mix(((unsigned long long)p[i] << 32) ^ (i * 0x9e3779b97f4a7c15ull));
+    for (int o = 32; o >= 1; o >>= 1) h += __shfl_xor(h, o, 64);
+    __shared__ unsigned long long part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
 inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -826,5 +853,15 @@ int et_launch_ratio_parts(const float *meas, const float *part, int D, size_t VP
 {
     hipLaunchKernelGGL(k_ratio_parts, dim3(nblk(VP, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), meas,
                        part, D, VP, eps, out);
+    return (int)hipGetLastError();
+}
+
+int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_stream_t stream)
+{
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(out), (size_t)2, 0.f);
+    const unsigned blocks = (unsigned)std::min<size_t>(nblk(n, 256), 2048);
+    hipLaunchKernelGGL(k_fingerprint, dim3(std::max(1u, blocks)), dim3(256), 0, st, reinterpret_cast<const unsigned *>(p),
+                       n, out);
     return (int)hipGetLastError();
 }

@@ -117,8 +117,14 @@ def batch_problem(i):
     shape = BATCH_SHAPES[int(rng.integers(len(BATCH_SHAPES)))]
     V = int(rng.integers(60, 181))
     K = int(rng.choice([7, 11, 15]))
+    return batch_problem_of(shape, V, K, 1000 + i)
+
+
+def batch_problem_of(shape, V, K, seed):
+    """A problem of the batched workload's recipe with a given volume shape,
+    view count and PSF size."""
     n_les = max(4, shape[2] // 16)
-    act, mu = phantom(shape, n_les, 1000 + i, lesion_axes=(2.0, max(4.0, shape[2] / 16.0)))
+    act, mu = phantom(shape, n_les, seed, lesion_axes=(2.0, max(4.0, shape[2] / 16.0)))
     s0, s1 = BATCH_PSF[K]
     return {"activity": act, "attenuation": mu, "psf": gaussian_psf(shape[0], K, s0, s1), "cameras": cameras(V),
             "counts": 1e6 * act.size / 64 ** 3}

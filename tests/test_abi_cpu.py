@@ -153,3 +153,120 @@ def test_bench_finds_its_traffic_rows():
         assert t is not None and 1e8 < t < 1e9, (cat, t)
     assert bench.measured_traffic("bp_pull") is not None
     assert bench.sweep_grid("bp_pull", 128, 128, 15, 16) is None
+
+
+def _no_gpu(L):
+    cnt = np.zeros(1, np.int32)
+    info = np.zeros(80, np.int32)
+    return L.et_array_list_gpus(cnt.ctypes.data_as(_lib.ip), info.ctypes.data_as(_lib.ip)) != 0
+
+
+def test_by_value_spellings_with_the_reference_binding():
+    """et_array_project / et_array_backproject bound with the argtypes of the
+    reference package (Python/NiftyRec/NiftyRec/NiftyRec.py:84,146) and called
+    with its argument conventions: parameter errors give 2 before any device
+    is touched, valid calls reach the device (8 here, where there is none)."""
+    P, c_float, c_int = ctypes.POINTER, ctypes.c_float, ctypes.c_int
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    for name in ("et_array_project", "et_array_backproject"):
+        f = getattr(L, name)
+        f.restype = c_int
+        f.argtypes = [P(c_float), P(c_int), P(c_float), P(c_int), P(c_float), P(c_int), P(c_float), P(c_int),
+                      P(c_float), P(c_int), c_float, c_float, c_int, c_int]
+    act = np.ones((16, 8, 16), np.float32)
+    cams = np.zeros((3, 4), np.float32).transpose()
+    csz = np.asarray(cams.shape, np.int32)
+    asz = np.asarray(act.shape, np.int32)
+    proj = np.zeros([16, 8, 4], np.float32)
+    psz = np.asarray(proj.shape, np.int32)
+    zero = np.zeros(3, np.int32)
+    fp, ip = P(c_float), P(c_int)
+
+    def project(psf, psf_size):
+        return L.et_array_project(act.ctypes.data_as(fp), asz.ctypes.data_as(ip), proj.ctypes.data_as(fp),
+                                  psz.ctypes.data_as(ip), cams.ctypes.data_as(fp), csz.ctypes.data_as(ip),
+                                  psf, psf_size.ctypes.data_as(ip), None, zero.ctypes.data_as(ip),
+                                  np.float32(0), np.float32(0), np.int32(1), np.int32(1))
+    even = np.ones((16, 4, 4), np.float32)
+    assert project(even.ctypes.data_as(fp), np.asarray(even.shape, np.int32)) == 2
+    bk = np.zeros([16, 8, 16], np.float32)
+    bsz = np.asarray(bk.shape, np.int32)
+    bad = np.asarray([16, 8, 5], np.int32)  # views != cameras
+    assert L.et_array_backproject(proj.ctypes.data_as(fp), bad.ctypes.data_as(ip), bk.ctypes.data_as(fp),
+                                  bsz.ctypes.data_as(ip), cams.ctypes.data_as(fp), csz.ctypes.data_as(ip), None,
+                                  zero.ctypes.data_as(ip), None, zero.ctypes.data_as(ip), np.float32(0),
+                                  np.float32(0), np.int32(1), np.int32(1)) == 2
+    if _no_gpu(_lib.load()):
+        assert project(None, zero) == 8
+
+
+def test_driver_parameter_errors_are_the_first_status(L):
+    """The EM drivers return the first non-zero status (the reference adds
+    them up): an even PSF is 2, not a sum of 2s."""
+    a = np.zeros((8, 6, 8), np.float32)
+    y = np.ones((4, 6, 8), np.float32)
+    psf = np.ones((8, 4, 4), np.float32)
+    for name, extra in (("et_array_mlem", ()), ("et_array_osem", (2,)), ("et_array_osem_ordered", (2,))):
+        f = getattr(L, name)
+        args = [a.ctypes.data_as(_lib.fp), 8, 6] + list(extra) + [y.ctypes.data_as(_lib.fp), 4, 0.0, 270.0, 1, 5, 0,
+                                                                  None, 1, psf.ctypes.data_as(_lib.fp), 4, 4, 0.0,
+                                                                  0.0, 1e-6, 1]
+        assert f(*args) == 2, name
+    # rotation axis beyond z
+    assert L.et_array_mlem(a.ctypes.data_as(_lib.fp), 8, 6, y.ctypes.data_as(_lib.fp), 4, 0.0, 270.0, 3, 5, 0, None,
+                           0, None, 0, 0, 0.0, 0.0, 1e-6, 1) == 2
+
+
+def test_gradient_two_d_branch_sizes():
+    """A [N, n_cameras, 1] sinogram describes a volume of one axial row: the
+    gradient must be [N, 1, N] (the reference reads the sinogram as
+    [gradient_size[0], gradient_size[1], n_cameras])."""
+    A = _lib_abi()
+    I, F = _lib.ip, _lib.fp
+    N, V = 8, 5
+    sino = np.ones(N * V, np.float32)
+    act = np.ones((N, 1, N), np.float32)
+    out = np.zeros_like(act)
+    cams = np.zeros(V, np.float32)
+
+    def call(gsize):
+        g = np.asarray(gsize, np.int32)
+        return A.et_array_gradient_attenuation(sino.ctypes.data_as(F), np.array([N, V, 1], np.int32).ctypes.data_as(I),
+                                               act.ctypes.data_as(F), g.ctypes.data_as(I), out.ctypes.data_as(F),
+                                               g.ctypes.data_as(I), cams.ctypes.data_as(F),
+                                               np.array([V, 1], np.int32).ctypes.data_as(I), None,
+                                               np.zeros(3, np.int32).ctypes.data_as(I), act.ctypes.data_as(F),
+                                               g.ctypes.data_as(I), 0.0, 0.0, 1, 1)
+    assert call([N, V, N]) == 2
+    if _no_gpu(_lib.load()):
+        assert call([N, 1, N]) == 8
+
+
+def test_plan_checks_tensor_shapes():
+    """Plan methods check every tensor's shape before the library call: a
+    wrong-sized tensor would make the kernels read or write out of bounds."""
+    torch = pytest.importorskip("torch")
+    from niftyrec_amd.api import Plan
+    p = Plan.__new__(Plan)
+    p.shape, p.sino_shape, p.device_index = (8, 6, 8), (4, 6, 8), 0
+    with pytest.raises(ValueError, match="shape"):
+        p._arg(torch.zeros(8, 6, 7), p.shape, "activity")
+    with pytest.raises(ValueError, match="shape"):
+        p._arg(torch.zeros(5, 6, 8), p.sino_shape, "out")
+    with pytest.raises(ValueError, match="float32"):
+        p._arg(torch.zeros(8, 6, 8, dtype=torch.float64), p.shape, "activity")
+    with pytest.raises(ValueError, match="contiguous"):
+        p._arg(torch.zeros(8, 8, 6).transpose(1, 2), p.shape, "activity")
+    with pytest.raises(ValueError, match="cuda:0"):
+        p._arg(torch.zeros(8, 6, 8), p.shape, "activity")
+    assert p._arg(None, p.shape, "attenuation") is None
+
+
+def test_niftyrec_module_without_a_device():
+    from niftyrec_amd import NiftyRec as NR
+    assert NR.et_is_block_multiple(37) == 1 and NR.et_get_block_size() == 1
+    if _no_gpu(_lib.load()):
+        assert NR.et_list_gpus() is None
+        act = np.ones((8, 6, 8), np.float32)
+        cams = np.zeros((3, 4), np.float32)
+        assert NR.et_project(act, cams) is None

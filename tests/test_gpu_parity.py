@@ -14,15 +14,11 @@ import ctypes
 import numpy as np
 import pytest
 
-from conftest import max_rel, rel_l2
+from conftest import RTOL_L2, max_rel, rel_l2, report_parity
 from niftyrec_amd import phantoms
 from oracle import orc, refabi
 
 pytestmark = pytest.mark.gpu
-
-RTOL_L2 = 1e-5
-RTOL_MAX = 1e-4
-
 
 def _case(shape=(32, 24, 32), V=16, K=5, seed=0, mu_scale=0.1):
     rng = np.random.default_rng(seed)
@@ -36,22 +32,7 @@ def _case(shape=(32, 24, 32), V=16, K=5, seed=0, mu_scale=0.1):
 
 
 def _report(name, got, ref32, ref64=None):
-    """relL2 <= 1e-5 against the reference; max_rel <= 1e-4 against the
-    reference, except where an FP64 result shows the reference itself further
-    than that from exact: then the HIP result's max_rel against FP64 may not
-    exceed the reference's own."""
-    e32, m32 = rel_l2(got, ref32), max_rel(got, ref32)
-    msg = "%s: relL2 %.2e max_rel %.2e vs reference" % (name, e32, m32)
-    if ref64 is not None:
-        msg += "; vs FP64 oracle: relL2 %.2e max_rel %.2e (reference itself: %.2e, %.2e)" % (
-            rel_l2(got, ref64), max_rel(got, ref64), rel_l2(ref32, ref64), max_rel(ref32, ref64))
-    print(msg)
-    assert e32 <= RTOL_L2, msg
-    if ref64 is None:
-        assert m32 <= RTOL_MAX, msg
-    else:
-        assert m32 <= RTOL_MAX or max_rel(got, ref64) <= max_rel(ref32, ref64) + 1e-6, msg
-        assert rel_l2(got, ref64) <= 2.0 * rel_l2(ref32, ref64) + 1e-7, msg
+    return report_parity(name, got, ref32, ref64)
 
 
 @pytest.mark.parametrize("use_mu", [True, False])

@@ -1,0 +1,193 @@
+"""The benchmark workloads at their full sizes on the GPU, against the oracle.
+
+spect128 (the benchmark's OSEM loop): 128^3 activity and attenuation, 120
+views, PSF 11, 8 ordered subsets, through exactly the path the benchmark
+times (Plan.osem with the timed depth chunking, the cached attenuation factors
+and subset sensitivities), compared with 8 successive oracle MLEM updates on
+views s, s + 8, ... (the update of emission-lib/_et_array_interface.cpp:1080-1131
+restricted to each subset), in single and double precision; plus the
+projection and reference backprojection of all 120 views.
+
+Differentiable projector at 128^3 x 120 views, PSF 11: the adjoint
+dot-product identity over all views, the exact attenuation gradient and the
+reference gradient against the oracle, and torch autograd of the Poisson
+negative log-likelihood against FP64 central differences.
+
+Batched reconstructions: one problem of each volume class of the batched
+workload (64^3, 96^3, 128^3, 160x96x160; PSF 15, 11, 7, 15) through one
+Batch.osem call, bit-identical to the plans run alone and matching the
+oracle's subset updates.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, report_parity
+from niftyrec_amd import phantoms
+from oracle import orc
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def _osem_oracle(y, mu, cams, psf, S, iterations=1, dtype=np.float32):
+    x = np.ones(mu.shape, dtype)
+    for _ in range(iterations):
+        for s in range(S):
+            x = orc.mlem_step(x, y[s::S], mu, cams[s::S], psf, dtype=dtype)
+    return x
+
+
+@pytest.fixture(scope="module")
+def spect128():
+    from niftyrec_amd.api import Plan
+    w = phantoms.workload("spect128")
+    act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    plan = Plan(act.shape, cams, psf)
+    A, M = _t(act), _t(mu)
+    y = phantoms.poisson_data(plan.project(A, M).cpu().numpy(), w["counts"], seed=1)
+    return w, plan, A, M, y
+
+
+def test_spect128_osem_iteration_bench_path(spect128):
+    """One OSEM iteration of the benchmark workload (8 subsets of 15 views)."""
+    w, plan, A, M, y = spect128
+    act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    X = torch.ones_like(A)
+    plan.osem(X, _t(y), M, subsets=8, iterations=1)
+    dp, db = plan.tuning()
+    print("spect128 OSEM: depth chunks projection %d, backprojection %d" % (dp, db))
+    report_parity("spect128 OSEM iteration (8 subsets)", X.cpu().numpy(), _osem_oracle(y, mu, cams, psf, 8),
+                  _osem_oracle(y, mu, cams, psf, 8, dtype=np.float64))
+
+
+def test_spect128_project_backproject_all_views(lib, spect128):
+    """Projection and reference backprojection of all 120 views through the
+    NiftyRec ABI (host buffers)."""
+    from oracle import refabi
+    w, plan, A, M, y = spect128
+    act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    st, p = refabi.project(lib, act, mu, cams, psf)
+    assert st == 0
+    report_parity("spect128 project (120 views)", p, orc.project(act, mu, cams, psf),
+                  orc.project(act, mu, cams, psf, dtype=np.float64))
+    st, b = refabi.backproject(lib, y, mu, cams, psf)
+    assert st == 0
+    report_parity("spect128 backproject (120 views)", b, orc.backproject(y, mu, cams, psf),
+                  orc.backproject(y, mu, cams, psf, dtype=np.float64))
+
+
+def test_spect128_adjoint_dot_product_all_views(spect128):
+    """<A x, y> = <x, A^T y> over all 120 views at 128^3 with PSF 11 and
+    attenuation (reductions in FP64)."""
+    from niftyrec_amd.api import ADJOINT
+    w, plan, A, M, _ = spect128
+    rng = np.random.default_rng(11)
+    x = rng.random(A.shape, dtype=np.float32)
+    yv = rng.random((plan.n_views,) + tuple(A.shape[1:]), dtype=np.float32)
+    Ax = plan.project(_t(x), M, truncate=False).cpu().numpy()
+    Aty = plan.backproject(_t(yv), M, mode=ADJOINT, truncate=False).cpu().numpy()
+    lhs = np.vdot(Ax.astype(np.float64), yv.astype(np.float64))
+    rhs = np.vdot(x.astype(np.float64), Aty.astype(np.float64))
+    mism = abs(lhs - rhs) / abs(lhs)
+    print("spect128 adjoint over 120 views: dot-product mismatch %.2e" % mism)
+    assert mism <= 1e-6
+
+
+SIX_VIEWS = [0, 19, 43, 60, 88, 111]
+
+
+def test_spect128_attenuation_gradients(spect128):
+    """Exact d<s, project(act, mu)>/d mu and the reference gradient at full
+    size on 6 of the 120 views, against the oracle."""
+    from niftyrec_amd.api import Plan
+    w, _, A, M, _ = spect128
+    act, mu, psf = w["activity"], w["attenuation"], w["psf"]
+    cams = w["cameras"][SIX_VIEWS]
+    plan = Plan(act.shape, cams, psf)
+    s = np.random.default_rng(12).random((len(cams),) + act.shape[1:], dtype=np.float32)
+    g = plan.gradient_attenuation(_t(s), A, M, exact=True, truncate=False).cpu().numpy()
+    ref = orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float64)
+    e = rel_l2(g, ref)
+    print("spect128 exact mu gradient (6 views): relL2 %.2e vs FP64 oracle" % e)
+    assert e <= 1e-5
+    g = plan.gradient_attenuation(_t(s), A, M, exact=False, truncate=True).cpu().numpy()
+    report_parity("spect128 reference gradient (6 views)", g, orc.gradient_ref(s, act, mu, cams, psf),
+                  orc.gradient_ref(s, act, mu, cams, psf, dtype=np.float64))
+
+
+def test_spect128_autograd_finite_differences(spect128):
+    """d/d activity and d/d mu of the Poisson NLL through torch autograd at
+    128^3 with PSF 11 (2 views), against FP64 central differences (step 1e-3)
+    on 64 random voxels each inside the body."""
+    from niftyrec_amd.api import Plan
+    from niftyrec_amd.torch_ops import poisson_nll, project
+    w, _, _, _, _ = spect128
+    act, mu, psf = w["activity"], w["attenuation"], w["psf"]
+    cams = w["cameras"][[7, 68]]
+    plan = Plan(act.shape, cams, psf)
+    y = orc.project(act * 1.2, mu * 0.9, cams, psf, dtype=np.float64)
+    y = (np.random.default_rng(13).poisson(y * 4.0) / 4.0).astype(np.float32)
+    Av = _t(act).requires_grad_(True)
+    Mv = _t(mu).requires_grad_(True)
+    loss = poisson_nll(project(Av, Mv, plan), _t(y))
+    loss.backward()
+    grads = (Av.grad.cpu().numpy(), Mv.grad.cpu().numpy())
+
+    def f(a, m):
+        p = orc.project(a, m, cams, psf, 0.0, 0.0, 0, dtype=np.float64)
+        return float((p - y * np.log(np.maximum(p, 1e-8))).sum())
+
+    rng = np.random.default_rng(14)
+    body = np.flatnonzero(mu.ravel() > 0)
+    h = 1e-3
+    base = [act.astype(np.float64), mu.astype(np.float64)]
+    for name, which in (("activity", 0), ("mu", 1)):
+        idx = rng.choice(body, 64, replace=False)
+        fd = []
+        for i in idx:
+            p, m = [b.copy() for b in base], [b.copy() for b in base]
+            p[which].ravel()[i] += h
+            m[which].ravel()[i] -= h
+            fd.append((f(*p) - f(*m)) / (2 * h))
+        fd = np.array(fd)
+        got = grads[which].ravel()[idx].astype(np.float64)
+        err = np.abs(got - fd).max() / np.abs(fd).max()
+        print("spect128 autograd d/d %s: max |autograd - FD| / max|FD| = %.2e over 64 voxels" % (name, err))
+        assert err <= 2e-4
+
+
+# One problem of each volume class of the batched workload: (nz, ny, nx), views, PSF size.
+BATCH_CASES = [((64, 64, 64), 150, 15), ((96, 96, 96), 100, 11), ((128, 128, 128), 60, 7),
+               ((160, 96, 160), 64, 15)]
+
+
+def test_batch_volume_classes():
+    """The four volume classes in one Batch.osem call (8 subsets, 1 iteration):
+    bit-identical to each plan's own et_plan_osem, and the oracle's 8
+    successive subset updates."""
+    from niftyrec_amd.api import Batch, Plan
+    probs = [phantoms.batch_problem_of(shape, V, K, 2000 + i) for i, (shape, V, K) in enumerate(BATCH_CASES)]
+    plans, Y, M = [], [], []
+    for w in probs:
+        p = Plan(w["activity"].shape, w["cameras"], w["psf"])
+        A, Mu = _t(w["activity"]), _t(w["attenuation"])
+        y = phantoms.poisson_data(p.project(A, Mu).cpu().numpy(), w["counts"], seed=4)
+        plans.append(p)
+        Y.append(y)
+        M.append(Mu)
+    Yd = [_t(y) for y in Y]
+    Xb = [torch.ones_like(m) for m in M]
+    Batch(plans).osem(Xb, Yd, M, subsets=8, iterations=1)
+    for i, (p, w) in enumerate(zip(plans, probs)):
+        Xs = torch.ones_like(M[i])
+        p.osem(Xs, Yd[i], M[i], subsets=8, iterations=1)
+        assert torch.equal(Xs, Xb[i]), "batch result differs from the plan run alone (problem %d)" % i
+        shape, V, K = BATCH_CASES[i]
+        mu, psf, cams = w["attenuation"], w["psf"], w["cameras"]
+        report_parity("batch %dx%dx%d V=%d K=%d OSEM iteration" % (shape[2], shape[1], shape[0], V, K),
+                      Xb[i].cpu().numpy(), _osem_oracle(Y[i], mu, cams, psf, 8),
+                      _osem_oracle(Y[i], mu, cams, psf, 8, dtype=np.float64))

@@ -139,6 +139,34 @@ def test_reference_gradient_closed_form():
         assert np.allclose(g[z], expect, rtol=1e-13)
 
 
+def test_reference_gradient_psf_after_the_sum():
+    """K9 semantics with a depth-dependent PSF at theta = 0: the PSF of plane z
+    blurs g(z) = s sum_{k>=z} act_k e^{-A_k} (A inclusive), after the sum
+    (emission-lib/_et.cpp:1829-1843 after
+    emission-lib_gpu/_et_attenuation_gradient_gpu_kernels.cu:24-39).  Built
+    here from numpy cumulative sums and scipy convolve2d with the full 2-D
+    planes; blurring the activity before the sum instead differs by far more
+    than the tolerance, so the test pins the placement."""
+    rng = np.random.default_rng(21)
+    nz, ny, nx = 9, 6, 9
+    act = rng.random((nz, ny, nx)) + 0.2
+    mu = 0.08 * rng.random((nz, ny, nx))
+    s = rng.random((1, ny, nx)) + 0.5
+    psf = phantoms.gaussian_psf(nz, 5, 0.5, 1.6).astype(np.float64)
+    g = orc.gradient_ref(s, act, mu, np.zeros(1), psf, 0.0, 0.0, 0, dtype=np.float64)
+    A = np.cumsum(mu, axis=0)
+    tail = np.cumsum((act * np.exp(-A))[::-1], axis=0)[::-1]
+    # the separated kernel of plane z (central row along x times central column
+    # along y over the centre value) as a full 2-D kernel
+    c = 2
+    k2d = [np.outer(psf[z][:, c] / psf[z][c, c], psf[z][c, :]) for z in range(nz)]
+    expect = np.stack([convolve2d(s[0] * tail[z], k2d[z], mode="same") for z in range(nz)])
+    assert np.allclose(g, expect, rtol=1e-12, atol=1e-14)
+    blurred_act = np.stack([convolve2d(act[z], k2d[z], mode="same") for z in range(nz)])
+    wrong = s[0][None] * np.cumsum((blurred_act * np.exp(-A))[::-1], axis=0)[::-1]
+    assert np.abs(wrong - expect).max() > 1e-3 * np.abs(expect).max()
+
+
 def _small_problem(seed=6, shape=(7, 5, 7), V=5, K=3):
     rng = np.random.default_rng(seed)
     act = rng.random(shape)
