@@ -4,13 +4,15 @@ Python/NiftyRec/NiftyRec/NiftyRec.py, bound to the HIP library.
 Same public functions, argument meaning and return conventions as the
 reference (et_project, et_backproject, et_set_gpu, et_list_gpus,
 et_is_block_multiple, et_get_block_size, et_osmapem_step, Reconstructor).
-Array sizes are passed as the numpy shapes, exactly as the reference does
-(NiftyRec.py:86-112), so a numpy array of shape (a, b, c) is read by the
-library as size [a, b, c] with the first size fastest; for the [N, m, N]
-volumes of a SPECT scan that is the numpy array indexed [z, y, x].  The
-returned projection has the reference's shape [a, b, n_cameras]; its memory
-holds the library's [x, y, camera] planes, i.e. it reads correctly as
-`proj.reshape(n_cameras, b, a)`.
+Array sizes are passed as the numpy shapes and the array memory as it is,
+exactly as the reference does (NiftyRec.py:86-112): an array of shape
+(a, b, c) is read by the library as size [a, b, c] with the first size
+fastest.  That is MATLAB (Fortran) order: x-fastest memory with the shape
+listed x first, e.g. a psf held as psf_zyx.T (shape (h, h, N)); for the
+[N, m, N] volumes of a SPECT scan a C-order array indexed [z, y, x] reads the
+same.  The returned projection has the reference's shape [a, b, n_cameras];
+its memory holds the library's [x, y, camera] planes, i.e. it reads
+correctly as `proj.reshape(n_cameras, b, a)`.
 
 Differences from the reference module, all fixes of Python-2-era bugs:
 cameras are always passed in [axis][camera] memory order (the reference
@@ -36,7 +38,13 @@ EPSILON = ET_EPSILON
 
 
 def _f32(a):
-    return np.ascontiguousarray(a, dtype=np.float32)
+    """float32 with the memory layout kept (the library reads the memory as
+    it is, sizes from the shape); arrays that are neither C- nor
+    Fortran-contiguous are copied to Fortran order, first size fastest."""
+    a = np.asarray(a, dtype=np.float32, order="K")
+    if not (a.flags.c_contiguous or a.flags.f_contiguous):
+        a = np.asfortranarray(a)
+    return a
 
 
 def _i32(a):
@@ -56,7 +64,7 @@ def _cameras(cameras):
     c = np.asarray(cameras)
     if c.ndim == 1:
         c = c.reshape(1, -1)
-    c = _f32(c)
+    c = np.ascontiguousarray(c, dtype=np.float32)
     return c, _i32([c.shape[1], c.shape[0]])
 
 
@@ -196,22 +204,22 @@ class Reconstructor:
         self.callback_updateactivity = function
 
     def set_activity(self, activity):
-        self.activity = np.float32(activity)
+        self.activity = np.asarray(activity, dtype=np.float32, order="K")
 
     def clear_activity(self):
         self.activity = np.ones(self.volume_size, dtype=np.float32)
 
     def set_attenuation(self, attenuation):
-        self.attenuation = np.float32(attenuation)
+        self.attenuation = np.asarray(attenuation, dtype=np.float32, order="K")
 
     def set_sinogram(self, sinogram):
         if sinogram.shape[1] != self.volume_size[0] or sinogram.shape[2] != self.volume_size[1]:
             print("Sinogram size not consistent with activity estimate, need reinterpolation")
             return
-        self.sinogram = np.float32(sinogram)
+        self.sinogram = np.asarray(sinogram, dtype=np.float32, order="K")
 
     def set_psf_matrix(self, psf):
-        self.psf = np.float32(psf)
+        self.psf = np.asarray(psf, dtype=np.float32, order="K")
 
     def set_cameras_matrix(self, cameras):
         self.cameras = np.float32(cameras)

@@ -52,6 +52,7 @@ def test_by_value_spellings_reference_conventions():
     camsT = cams.transpose()
     cams_size = np.asarray(camsT.shape, np.int32)
     act_size = np.asarray(act.shape, np.int32)
+    psf = psf.T  # MATLAB order: shape (h, h, N), x-fastest memory
     psf_size = np.asarray(psf.shape, np.int32)
     mu_size = np.asarray(mu.shape, np.int32)
     proj = np.zeros([act_size[0], act_size[1], cams_size[0]], np.float32)
@@ -63,7 +64,7 @@ def test_by_value_spellings_reference_conventions():
     assert st == 0
     V, (nz, ny, nx) = len(cams1), act.shape
     report_parity("et_array_project (reference binding)", proj.reshape(V, ny, nx),
-                  orc.project(act, mu, cams, psf), orc.project(act, mu, cams, psf, dtype=np.float64))
+                  orc.project(act, mu, cams, psf.T), orc.project(act, mu, cams, psf.T, dtype=np.float64))
     sino = np.ascontiguousarray(proj)
     bk = np.zeros([proj_size[0], proj_size[1], proj_size[0]], np.float32)
     bk_size = np.asarray(bk.shape, np.int32)
@@ -74,8 +75,8 @@ def test_by_value_spellings_reference_conventions():
                                 np.int32(1))
     assert st == 0
     s3 = proj.reshape(V, ny, nx)
-    report_parity("et_array_backproject (reference binding)", bk, orc.backproject(s3, mu, cams, psf),
-                  orc.backproject(s3, mu, cams, psf, dtype=np.float64))
+    report_parity("et_array_backproject (reference binding)", bk, orc.backproject(s3, mu, cams, psf.T),
+                  orc.backproject(s3, mu, cams, psf.T, dtype=np.float64))
     # absent psf and attenuation: size arrays of zeros, null pointers
     zeros3 = np.zeros(3, np.int32)
     proj2 = np.zeros_like(proj)
@@ -95,16 +96,17 @@ def test_niftyrec_module_project_backproject():
     cams = np.zeros((3, len(cams1)), np.float32)
     cams[1] = cams1
     V, (nz, ny, nx) = len(cams1), act.shape
-    proj = NR.et_project(act, cams, psf, mu)
+    proj = NR.et_project(act, cams, psf.T, mu)  # psf in MATLAB order, (h, h, N)
     assert proj is not None and proj.shape == (nz, ny, V)
     report_parity("NiftyRec.et_project", proj.reshape(V, ny, nx), orc.project(act, mu, cams, psf),
                   orc.project(act, mu, cams, psf, dtype=np.float64))
-    bk = NR.et_backproject(proj, cams, psf, mu)
+    bk = NR.et_backproject(proj, cams, psf.T, mu)
     assert bk is not None and bk.shape == (nz, ny, nx)
     s3 = proj.reshape(V, ny, nx)
     report_parity("NiftyRec.et_backproject", bk, orc.backproject(s3, mu, cams, psf),
                   orc.backproject(s3, mu, cams, psf, dtype=np.float64))
     assert NR.et_project(act, cams, np.ones((4, 4, nz), np.float32), mu) is None  # even PSF: status 2 -> None
+    assert NR.et_project(act, cams, psf, mu) is None  # (N, h, h) read as size [N, h, h]: status 2 -> None
     n, info = NR.et_list_gpus()
     assert n >= 1 and len(info) == NR.SIZE_INFO * min(n, NR.MAX_DEVICES)
 
@@ -136,13 +138,13 @@ def test_niftyrec_osmapem_step_and_reconstructor():
     y = orc.project(act, mu, cams, psf)  # [camera, y, x]
     x0 = np.ones_like(act)
     r = np.random.default_rng(43).random(4)
-    got = NR.et_osmapem_step(4, x0, y, cams, mu, psf, 0.0, 0.0, 1, rng=np.random.default_rng(43))
+    got = NR.et_osmapem_step(4, x0, y, cams, mu, psf.T, 0.0, 0.0, 1, rng=np.random.default_rng(43))
     report_parity("NiftyRec.et_osmapem_step", got, _osmapem_oracle(4, x0, y, cams, mu, psf, r))
 
     rec = NR.Reconstructor(act.shape)
     rec.set_cameras_matrix(cams)
     rec.set_sinogram(y)
-    rec.set_psf_matrix(psf)
+    rec.set_psf_matrix(psf.T)
     rec.set_attenuation(mu)
     np.random.seed(44)
     assert rec.reconstruct("osem", {"steps": 2, "subset_order": 4}, block=True)
