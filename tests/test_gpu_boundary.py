@@ -222,7 +222,7 @@ def test_driver_general_path_and_status():
     act, mu, _, psf = _case(shape=(24, 16, 24), V=8, K=3, seed=45)
     cams3 = api.make_cameras(0.0, 315.0, 8, 0)
     y = orc.project(act, mu, cams3, psf)
-    got = api.osem_ordered(y, 2, 0.0, 315.0, 1, psf=psf, attenuation=mu)
+    got = api.osem_ordered(y, 2, 0.0, 315.0, 1, psf=psf, attenuation=mu, rotation_axis=0)
     x = np.ones_like(act)
     for s in range(2):
         x = orc.mlem_step(x, y[s::2], mu, cams3[:, s::2], psf)
