@@ -59,6 +59,27 @@ __device__ __forceinline__ float exp_neg(float A)
     return __builtin_amdgcn_exp2f(x) * fmaf(r, LN2, 1.f);
 }
 
+// Error-free transformation of a sum (Knuth's TwoSum): s + e == a + b exactly,
+// s = fl(a + b), for any a, b.  Six VALU operations; built without
+// floating-point contraction (the file's contract setting is not relied on).
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e)
+{
+#pragma clang fp contract(off)
+    s = a + b;
+    const float bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated running sum (hi, lo): hi + lo carries the sum of every term to
+// about twice the working precision (Neumaier/TwoSum accumulation).
+__device__ __forceinline__ void comp_add(float &hi, float &lo, float x)
+{
+    float s, e;
+    two_sum(hi, x, s, e);
+    hi = s;
+    lo += e;
+}
+
 // Loads and stores through the global address space.  Pointers read from an
 // argument record in memory (the grouped launches) are generic to the
 // compiler, which would otherwise emit flat accesses.
@@ -357,22 +378,22 @@ __device__ __forceinline__ Samp fast_sample(const FastView &fv, float fxp, int z
     return make_samp(px, pz, nx, nz, dx, dz);
 }
 
-// Sum over d < D of part[d * stride + i], added in order of d.  The loads go out
-// eight at a time: a loop that adds each value as it arrives waits out one
-// memory latency per term.
+// Sum over d < D of part[d * stride + i], added in order of d with TwoSum
+// compensation.  The loads go out eight at a time: a loop that adds each value
+// as it arrives waits out one memory latency per term.
 __device__ __forceinline__ float sum_parts(const float *part, int D, size_t stride, size_t i)
 {
-    float p = 0.f;
+    float p = 0.f, lo = 0.f;
     int d = 0;
     for (; d + 8 <= D; d += 8) {
         float v[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) v[k] = gload(part + (size_t)(d + k) * stride + i);
 #pragma unroll
-        for (int k = 0; k < 8; k++) p += v[k];
+        for (int k = 0; k < 8; k++) comp_add(p, lo, v[k]);
     }
-    for (; d < D; d++) p += gload(part + (size_t)d * stride + i);
-    return p;
+    for (; d < D; d++) comp_add(p, lo, gload(part + (size_t)d * stride + i));
+    return p + lo;
 }
 
 // Input projection value s(v, y, x) for the backprojection sweeps.

@@ -234,7 +234,8 @@ struct et_plan {
     int gy = ET_GY, gmu = ET_GY, yp = 0, xp = 0, zp = 0, yu = 0, slabs = 0;
     size_t yf_elems = 0, ustride = 0, P = 0, nvox = 0;
     // workspaces
-    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, astart, atot, tmp, tmp2, em_sino, em_vol, att;
+    DevBuf act_yf, mu_yf, rot_act, rot_mu, u, part, ratio, csum, csum_lo, astart, atot, atot_lo, tmp, tmp2, em_sino,
+        em_vol, att;
     // OSEM caches (valid while osem_valid; keyed on the subset count, the
     // chunking, eps and the attenuation map's key: its content fingerprint, or
     // its address when mu_check is off)
@@ -509,16 +510,21 @@ void mu_prefix(et_plan *p, const float *recs, int nv, int D, const float *mu_yf,
     a.zlen = (p->nz + D - 1) / D;
     a.mu = mu_yf;
     p->csum.ensure((size_t)nv * D * p->P, false);
+    p->csum_lo.ensure((size_t)nv * D * p->P, false);
     a.part = p->csum.p;
+    a.part_lo = p->csum_lo.p;
     {
-        ProfScope ps(p, ET_PROF_MUSUM, (double)nv * (4.0 * p->nvox + 4.0 * D * p->P));
+        ProfScope ps(p, ET_PROF_MUSUM, (double)nv * (4.0 * p->nvox + 8.0 * D * p->P));
         check_launch(et_launch_sweep(ET_MODE_MUSUM, 0, 1, 0, a, (et_stream_t)p->st));
     }
-    float *as = nullptr, *at = nullptr;
+    float *as = nullptr, *at = nullptr, *atl = nullptr;
     if (want_start) as = astart_out ? astart_out : p->astart.ensure((size_t)nv * D * p->P, false);
-    if (want_tot) at = atot_out ? atot_out : p->atot.ensure((size_t)nv * p->P, false);
+    if (want_tot) {
+        at = atot_out ? atot_out : p->atot.ensure((size_t)nv * p->P, false);
+        atl = p->atot_lo.ensure((size_t)nv * p->P, false);
+    }
     ProfScope ps(p, ET_PROF_OTHER, 0);
-    check_launch(et_launch_chunk_scan(p->csum.p, as, at, nv, D, (int)p->P, (et_stream_t)p->st));
+    check_launch(et_launch_chunk_scan(p->csum.p, p->csum_lo.p, as, at, atl, nv, D, (int)p->P, (et_stream_t)p->st));
 }
 
 // Floats of the attenuation-factor buffer of nv views ([nv][nz][yu][roundup(nx, 64)]).
@@ -732,7 +738,8 @@ void op_project(et_plan *p, const float *act, const float *mu, float *sino, int 
         if (!act) {
             mu_prefix(p, p->d_recs.p, p->V, 1, mu_yf, false, true);
             ProfScope ps(p, ET_PROF_OTHER, 0);
-            check_launch(et_launch_bg_only(p->atot.p, p->bg, (int)(p->V * p->P), sino, truncate, (et_stream_t)p->st));
+            check_launch(et_launch_bg_only(p->atot.p, p->atot_lo.p, p->bg, (int)(p->V * p->P), sino, truncate,
+                                           (et_stream_t)p->st));
             return;
         }
         const int D = mu ? choose_D(p, p->V) : std::max(1, choose_D(p, p->V));
@@ -751,7 +758,8 @@ void op_project(et_plan *p, const float *act, const float *mu, float *sino, int 
         if (mu) rotate_general(p, mu, v, p->bga, p->rot_mu, false, false);
         if (!act) {
             mu_prefix(p, p->d_ident.p, 1, 1, p->rot_mu.p, false, true);
-            check_launch(et_launch_bg_only(p->atot.p, p->bg, (int)p->P, sv, truncate, (et_stream_t)p->st));
+            check_launch(et_launch_bg_only(p->atot.p, p->atot_lo.p, p->bg, (int)p->P, sv, truncate,
+                                           (et_stream_t)p->st));
             continue;
         }
         rotate_general(p, act, v, p->bg, p->rot_act, true, false);
@@ -836,6 +844,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
             a.act = p->act_yf.p;
             a.mu = p->mu_yf.p;
             a.atot = p->atot.p;
+            a.atot_lo = p->atot_lo.p;
             a.src = ET_SRC_SINO;
             a.sino = sino + (size_t)v0 * p->P;
             a.u = p->u.p;
@@ -862,6 +871,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
         a.bg_act = 0.f;
         a.bg_mu = 0.f;
         a.atot = p->atot.p;
+        a.atot_lo = p->atot_lo.p;
         a.src = ET_SRC_SINO;
         a.sino = sino + (size_t)v * p->P;
         a.u = p->u.p;

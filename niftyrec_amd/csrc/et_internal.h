@@ -62,7 +62,9 @@ struct EtSweepArgs {
     float bg_line;           // background term of the line integral
     const float *astart;     // [V][D][ny][nx] attenuation at the start of each chunk (D > 1)
     const float *atot;       // [V][ny][nx] total attenuation along each ray (reverse sweeps)
+    const float *atot_lo;    // its low parts: atot + atot_lo is the compensated sum
     float *part;             // PROJECT: [D][V][ny][nx]; MUSUM: [V][D][ny][nx]
+    float *part_lo;          // MUSUM: low parts of the compensated chunk sums
     int truncate;            // PROJECT with D == 1: clamp negatives
     int src;                 // EtSource
     const float *sino;       // ET_SRC_SINO: [V][ny][nx]
@@ -126,11 +128,13 @@ int et_launch_unpack_yf(const float *yf, float *abi, int nx, int ny, int nz, int
 int et_launch_resample_general(const float *src, int nx, int ny, int nz, const float *M12,
                                float bg, float *dst, int dst_yf, int yp, int gy,
                                et_stream_t stream);
-int et_launch_chunk_scan(const float *csum, float *astart, float *atot, int V, int D, int P,
-                         et_stream_t stream);
+// Attenuation at the start of every chunk (astart, rounded) and along whole
+// rays (atot + atot_lo), compensated sums of the chunk sums csum + csum_lo.
+int et_launch_chunk_scan(const float *csum, const float *csum_lo, float *astart, float *atot, float *atot_lo,
+                         int V, int D, int P, et_stream_t stream);
 int et_launch_reduce_parts(const float *part, int D, int V, int P, float *sino, int truncate,
                            et_stream_t stream);
-int et_launch_bg_only(const float *atot, float bg, int n, float *sino, int truncate,
+int et_launch_bg_only(const float *atot, const float *atot_lo, float bg, int n, float *sino, int truncate,
                       et_stream_t stream);
 int et_launch_conv_axis(const float *src, float *dst, int nx, int ny, int nz, long sx, long sy,
                         long sz, long off, int axis, const float *sep, int K, int flip,

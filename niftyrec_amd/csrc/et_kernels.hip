@@ -459,21 +459,26 @@ __global__ void __launch_bounds__(256) k_unpack_yf(const float *__restrict__ yf,
     }
 }
 
-// astart[v][d] = sum_{d' < d} csum[v][d'], atot[v] = sum_d csum[v][d].
-__global__ void __launch_bounds__(256) k_chunk_scan(const float *__restrict__ csum, float *__restrict__ astart,
-                                                    float *__restrict__ atot, int V, int D, int P)
+// astart[v][d] = sum_{d' < d} csum[v][d'], atot[v] = sum_d csum[v][d], with
+// csum + csum_lo the compensated chunk sums; the scan is compensated as well and
+// the whole-ray total kept as a (hi, lo) pair.
+__global__ void __launch_bounds__(256) k_chunk_scan(const float *__restrict__ csum, const float *__restrict__ csum_lo,
+                                                    float *__restrict__ astart, float *__restrict__ atot,
+                                                    float *__restrict__ atot_lo, int V, int D, int P)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)V * P) return;
     const int v = (int)(i / P);
     const size_t r = i % P;
-    float run = 0.f;
+    float hi = 0.f, lo = 0.f;
     for (int d = 0; d < D; d++) {
         const size_t k = ((size_t)v * D + d) * P + r;
-        if (astart) astart[k] = run;
-        run += csum[k];
+        if (astart) astart[k] = hi + lo;
+        comp_add(hi, lo, csum[k]);
+        lo += csum_lo[k];
     }
-    if (atot) atot[i] = run;
+    if (atot) atot[i] = hi;
+    if (atot_lo) atot_lo[i] = lo;
 }
 
 __global__ void __launch_bounds__(256) k_reduce_parts(const float *__restrict__ part, int D, size_t VP,
@@ -486,12 +491,12 @@ __global__ void __launch_bounds__(256) k_reduce_parts(const float *__restrict__ 
     sino[i] = s;
 }
 
-__global__ void __launch_bounds__(256) k_bg_only(const float *__restrict__ atot, float bg, size_t n,
-                                                 float *__restrict__ sino, int truncate)
+__global__ void __launch_bounds__(256) k_bg_only(const float *__restrict__ atot, const float *__restrict__ atot_lo,
+                                                 float bg, size_t n, float *__restrict__ sino, int truncate)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float s = bg * exp_neg(atot[i]);
+    float s = bg * exp_neg(atot[i] + atot_lo[i]);
     if (truncate && s < 0.f) s = 0.f;
     sino[i] = s;
 }
@@ -774,10 +779,11 @@ int et_launch_resample_general(const float *src, int nx, int ny, int nz, const f
     return (int)hipGetLastError();
 }
 
-int et_launch_chunk_scan(const float *csum, float *astart, float *atot, int V, int D, int P, et_stream_t stream)
+int et_launch_chunk_scan(const float *csum, const float *csum_lo, float *astart, float *atot, float *atot_lo,
+                         int V, int D, int P, et_stream_t stream)
 {
     hipLaunchKernelGGL(k_chunk_scan, dim3(nblk((size_t)V * P, 256)), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), csum, astart, atot, V, D, P);
+                       reinterpret_cast<hipStream_t>(stream), csum, csum_lo, astart, atot, atot_lo, V, D, P);
     return (int)hipGetLastError();
 }
 
@@ -789,10 +795,11 @@ int et_launch_reduce_parts(const float *part, int D, int V, int P, float *sino, 
     return (int)hipGetLastError();
 }
 
-int et_launch_bg_only(const float *atot, float bg, int n, float *sino, int truncate, et_stream_t stream)
+int et_launch_bg_only(const float *atot, const float *atot_lo, float bg, int n, float *sino, int truncate,
+                      et_stream_t stream)
 {
     hipLaunchKernelGGL(k_bg_only, dim3(nblk((size_t)n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                       atot, bg, (size_t)n, sino, truncate);
+                       atot, atot_lo, bg, (size_t)n, sino, truncate);
     return (int)hipGetLastError();
 }
 

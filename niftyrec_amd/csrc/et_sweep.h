@@ -38,12 +38,23 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
     float *tab = lds + (R > 0 ? c.bufsz : 0) + wv * ET_COOP_TAB;
     float *stage = R > 0 ? lds + (64 * wv + R) * L : lds + (c.nthr >> 6) * ET_COOP_TAB + wv * 64 * (LA + 4);
 
-    float A[TS], acc[TS];
+    // Compensated sums along the ray: the attenuation A as a TwoSum (hi, lo)
+    // pair, the projection acc with Kahan's correction accc (true sum about
+    // acc - accc).
+    float A[TS], Al[TS], acc[TS], accc[TS];
 #pragma unroll
     for (int i = 0; i < TS; i++) {
         A[i] = 0.f;
+        Al[i] = 0.f;
         acc[i] = 0.f;
+        accc[i] = 0.f;
     }
+    auto kahan = [&](int i, float h, float e) {
+        const float y = fmaf(h, e, -accc[i]);
+        const float t = acc[i] + y;
+        accc[i] = (t - acc[i]) - y;
+        acc[i] = t;
+    };
     if (HAS_MU && AM != 2 && c.d > 0 && c.xin) {
         const float *as = a.astart + ((size_t)c.v * a.D + c.d) * c.P;
 #pragma unroll
@@ -70,7 +81,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
         if constexpr (HAS_MU && AM != 2) {
             // attenuation up to and including plane z (used after the row pass)
             auto cum = [&](int i, float m) {
-                A[i] += m;
+                comp_add(A[i], Al[i], m);
                 return 0.f;
             };
             if constexpr (COOP) {
@@ -78,7 +89,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
                 coop_rows<TS, TS, 0, OPM>(a.mu + (a.gmu + c.y0), c.dx, c.dz, s, tab, stage, cum, dummy);
             } else {
                 bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s,
-                                        [&](int i, float m) { A[i] += m; });
+                                        [&](int i, float m) { comp_add(A[i], Al[i], m); });
             }
         }
         float h[TS];
@@ -98,16 +109,16 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
         if constexpr (HAS_MU && AM == 1) {
 #pragma unroll
             for (int i = 0; i < TS; i++) {
-                const float e = exp_neg(A[i]);
-                acc[i] = fmaf(h[i], e, acc[i]);
+                const float e = exp_neg(A[i] + Al[i]);
+                kahan(i, h[i], e);
                 gstore(ez + i * c.nthr, e);
             }
         } else if constexpr (HAS_MU && AM == 2) {
 #pragma unroll
-            for (int i = 0; i < TS; i++) acc[i] = fmaf(h[i], gload(ez + i * c.nthr), acc[i]);
+            for (int i = 0; i < TS; i++) kahan(i, h[i], gload(ez + i * c.nthr));
         } else {
 #pragma unroll
-            for (int i = 0; i < TS; i++) acc[i] = HAS_MU ? fmaf(h[i], exp_neg(A[i]), acc[i]) : acc[i] + h[i];
+            for (int i = 0; i < TS; i++) kahan(i, h[i], HAS_MU ? exp_neg(A[i] + Al[i]) : 1.f);
         }
     }
     if (c.d == a.D - 1) {
@@ -116,11 +127,11 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
             if (a.bg_line != 0.f) {
                 const float *el = a.att + (((size_t)c.v * a.nz + a.nz - 1) * a.yu + c.y0) * c.nthr + c.tx;
 #pragma unroll
-                for (int i = 0; i < TS; i++) acc[i] += a.bg_line * el[i * c.nthr];
+                for (int i = 0; i < TS; i++) kahan(i, a.bg_line, el[i * c.nthr]);
             }
         } else {
 #pragma unroll
-            for (int i = 0; i < TS; i++) acc[i] += HAS_MU ? a.bg_line * exp_neg(A[i]) : a.bg_line;
+            for (int i = 0; i < TS; i++) kahan(i, a.bg_line, HAS_MU ? exp_neg(A[i] + Al[i]) : 1.f);
         }
     }
     if (c.xin) {
@@ -129,7 +140,7 @@ __device__ __forceinline__ void project_body(const EtSweepArgs &a, const Blk &b)
         for (int i = 0; i < TS; i++) {
             const int y = c.y0 + i;
             if (y < a.ny) {
-                float val = acc[i];
+                float val = acc[i] - accc[i];
                 if (a.truncate && val < 0.f) val = 0.f;
                 gstore(out + (size_t)y * a.nx + c.tx, val);
             }
@@ -284,14 +295,20 @@ __global__ void __launch_bounds__(512) k_grad_ref(EtSweepArgs a)
     zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
     const int ys = c.y0 - R;
 
-    float sv[NA], At[NA], suf[NA], tail[NA];
+    // A(z') = A_tot - sum_{k>z'} mu'(k) with both terms as compensated (hi, lo)
+    // pairs: near the detector A is small against A_tot, and a plain float
+    // difference would carry an absolute error of about eps * A_tot there
+    float sv[NA], At[NA], Atl[NA], suf[NA], sufl[NA], tail[NA];
 #pragma unroll
     for (int j = 0; j < NA; j++) {
         const int y = ys + j;
         const bool ok = c.xin && y >= 0 && y < a.ny;
         sv[j] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
-        At[j] = ok ? a.atot[(size_t)c.v * c.P + (size_t)y * a.nx + c.tx] : 0.f;
+        const size_t k = (size_t)c.v * c.P + (size_t)y * a.nx + c.tx;
+        At[j] = ok ? a.atot[k] : 0.f;
+        Atl[j] = ok ? a.atot_lo[k] : 0.f;
         suf[j] = 0.f;
+        sufl[j] = 0.f;
         tail[j] = 0.f;
     }
     for (int z = a.nz - 1; z >= 0; z--) {
@@ -300,8 +317,8 @@ __global__ void __launch_bounds__(512) k_grad_ref(EtSweepArgs a)
         // mu rows first: tail needs A(z') before this plane's mu joins the suffix
         float Az[NA];
         bilinear_map<NA, LA, SA>(a.mu + s.off + (a.gmu + c.y0 - R4), c.dx, c.dz, s, [&](int j, float m) {
-            Az[j] = At[j] - suf[j];
-            suf[j] += m;
+            Az[j] = (At[j] - suf[j]) + (Atl[j] - sufl[j]);
+            comp_add(suf[j], sufl[j], m);
         });
         float yc[TS];
         resample_column<TS, R, NA, LA, SA>(a.act + s.off + (a.gy + c.y0 - R4), c.dx, c.dz, s, kz + K,
@@ -333,15 +350,19 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
     const SweepCtx c = sweep_ctx<TS, R>(a, false, blk_here());
     zero_lds<R>(lds, 2 * c.bufsz, c.tx, c.nthr);
 
-    float sv[TS], At[TS], suf[TS], tail[TS];
+    // A(z') = A_tot - sum_{k>z'} mu'(k) in compensated (hi, lo) pairs, as in k_grad_ref
+    float sv[TS], At[TS], Atl[TS], suf[TS], sufl[TS], tail[TS];
 #pragma unroll
     for (int i = 0; i < TS; i++) {
         const int y = c.y0 + i;
         const bool ok = c.xin && y < a.ny;
         sv[i] = ok ? load_source(a, c.v, y, c.tx) : 0.f;
-        At[i] = ok ? a.atot[(size_t)c.v * c.P + (size_t)y * a.nx + c.tx] : 0.f;
+        const size_t k = (size_t)c.v * c.P + (size_t)y * a.nx + c.tx;
+        At[i] = ok ? a.atot[k] : 0.f;
+        Atl[i] = ok ? a.atot_lo[k] : 0.f;
         suf[i] = 0.f;
-        tail[i] = a.bg_line * exp_neg(At[i]);
+        sufl[i] = 0.f;
+        tail[i] = a.bg_line * exp_neg(At[i] + Atl[i]);
     }
     for (int z = a.nz - 1; z >= 0; z--) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
@@ -364,8 +385,8 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
         }
         float out[TS];
         bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s, [&](int i, float m) {
-            const float Az = At[i] - suf[i];
-            suf[i] += m;
+            const float Az = (At[i] - suf[i]) + (Atl[i] - sufl[i]);
+            comp_add(suf[i], sufl[i], m);
             tail[i] = fmaf(h[i], exp_neg(Az), tail[i]);
             out[i] = -sv[i] * tail[i];
         });
@@ -373,24 +394,32 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
     }
 }
 
-// MUSUM: part[v][d][y][x] = sum over chunk d of mu'(x, y, z').
+// MUSUM: part[v][d][y][x] = sum over chunk d of mu'(x, y, z'), accumulated as
+// a compensated (hi, lo) pair; the low parts go to part_lo (same layout).
 template <int TS>
 __global__ void __launch_bounds__(512) k_musum(EtSweepArgs a)
 {
     const SweepCtx c = sweep_ctx<TS, 0>(a, true, blk_here());
     if (!c.xin) return;
-    float acc[TS];
+    float acc[TS], lo[TS];
 #pragma unroll
-    for (int i = 0; i < TS; i++) acc[i] = 0.f;
+    for (int i = 0; i < TS; i++) {
+        acc[i] = 0.f;
+        lo[i] = 0.f;
+    }
     for (int z = c.zb; z < c.ze; z++) {
         const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
-        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s, [&](int i, float m) { acc[i] += m; });
+        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s,
+                                [&](int i, float m) { comp_add(acc[i], lo[i], m); });
     }
-    float *out = a.part + ((size_t)c.v * a.D + c.d) * c.P;
+    const size_t off = ((size_t)c.v * a.D + c.d) * c.P;
 #pragma unroll
     for (int i = 0; i < TS; i++) {
         const int y = c.y0 + i;
-        if (y < a.ny) out[(size_t)y * a.nx + c.tx] = acc[i];
+        if (y < a.ny) {
+            a.part[off + (size_t)y * a.nx + c.tx] = acc[i];
+            a.part_lo[off + (size_t)y * a.nx + c.tx] = lo[i];
+        }
     }
 }
 
