@@ -214,7 +214,8 @@ def main_joint(args):
     M = (Mtrue * 0.8).contiguous()
 
     def step():
-        plan.set_tuning(16, 0)            # mu changed: rebuild the cached sensitivities
+        # M changed in place (its version advanced): osem rebuilds the cached
+        # sensitivities and attenuation factors
         plan.osem(X, Y, M, subsets=S, iterations=1)
         Mv = M.clone().requires_grad_(True)
         loss = poisson_nll(project(X, Mv, plan), Y)

@@ -245,9 +245,17 @@ class Plan:
     Tensors passed to the methods must be float32, contiguous, of the plan's
     volume or sinogram shape and on the plan's device (checked before every
     call: the kernels trust the sizes); work is queued on torch's current
-    stream."""
+    stream.
 
-    def __init__(self, shape, cameras, psf=None, background=0.0, background_attenuation=0.0):
+    mu_key: how osem knows that the attenuation map changed, so that its cached
+    subset sensitivities and attenuation factors are rebuilt.  "version"
+    (default): the tensor itself and its autograd version counter, which every
+    in-place torch operation (and every write of this module into an `out`
+    tensor) advances; no device work.  "content": a device fingerprint of the
+    map at every call (one pass over it and one stream synchronisation), for
+    maps written behind torch's back."""
+
+    def __init__(self, shape, cameras, psf=None, background=0.0, background_attenuation=0.0, mu_key="version"):
         L = _lib.load()
         nz, ny, nx = shape
         self.shape = tuple(int(v) for v in shape)
@@ -265,6 +273,11 @@ class Plan:
         dev = ctypes.c_int()
         _lib.check(L.et_plan_get_device(handle, ctypes.byref(dev)), "et_plan_get_device")
         self.device_index = dev.value
+        if mu_key not in ("version", "content"):
+            raise ValueError("mu_key must be 'version' or 'content'")
+        self.mu_key = mu_key
+        self._mu_seen = None  # (weak reference, version) of the map of the cached OSEM data
+        _lib.check(L.et_plan_set_mu_check(handle, int(mu_key == "content")), "et_plan_set_mu_check")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -311,6 +324,27 @@ class Plan:
         import torch
         return torch.empty(shape, dtype=torch.float32, device=like.device)
 
+    @staticmethod
+    def _written(t):
+        """Advance the version counter of a tensor this module wrote in place."""
+        import torch
+        if hasattr(torch.autograd.graph, "increment_version"):
+            torch.autograd.graph.increment_version(t)
+
+    def _track_mu(self, mu):
+        """With mu_key "version": drop the cached OSEM data unless `mu` is the
+        tensor, at the same version, that they were built from."""
+        if self.mu_key != "version":
+            return
+        import weakref
+        seen = self._mu_seen
+        same = (seen is not None and
+                (mu is None and seen[0] is None or
+                 mu is not None and seen[0] is not None and seen[0]() is mu and seen[1] == mu._version))
+        if not same:
+            _lib.check(self._L.et_plan_invalidate(self._h), "et_plan_invalidate")
+        self._mu_seen = (None, None) if mu is None else (weakref.ref(mu), mu._version)
+
     # -- operators ------------------------------------------------------------
     def project(self, activity, attenuation=None, truncate=True, out=None):
         if activity is None and attenuation is None:
@@ -322,6 +356,7 @@ class Plan:
         o = self._arg(out, self.sino_shape, "out")
         st = self._L.et_plan_project(self._h, a, m, o, int(bool(truncate)), self._stream())
         _lib.check(st, "et_plan_project")
+        self._written(out)
         return out
 
     def backproject(self, sinogram, attenuation=None, mode=REFERENCE, truncate=True, out=None):
@@ -331,6 +366,7 @@ class Plan:
         o = self._arg(out, self.shape, "out")
         st = self._L.et_plan_backproject(self._h, s, m, o, int(mode), int(bool(truncate)), self._stream())
         _lib.check(st, "et_plan_backproject")
+        self._written(out)
         return out
 
     def gradient_attenuation(self, sinogram, activity, attenuation, exact=False, truncate=True, out=None):
@@ -344,6 +380,7 @@ class Plan:
         st = self._L.et_plan_gradient_attenuation(self._h, s, a, m, o, int(bool(exact)), int(bool(truncate)),
                                                   self._stream())
         _lib.check(st, "et_plan_gradient_attenuation")
+        self._written(out)
         return out
 
     def osem(self, activity, sinogram, attenuation=None, subsets=1, iterations=1, epsilon=1e-6):
@@ -354,18 +391,22 @@ class Plan:
         a = self._arg(activity, self.shape, "activity")
         s = self._arg(sinogram, self.sino_shape, "sinogram")
         m = self._arg(attenuation, self.shape, "attenuation")
+        self._track_mu(attenuation)
         st = self._L.et_plan_osem(self._h, a, s, m, int(subsets), int(iterations), float(epsilon), self._stream())
         _lib.check(st, "et_plan_osem")
+        self._written(activity)
         return activity
 
     def set_tuning(self, depth_chunks=0, view_batch=0):
         _lib.check(self._L.et_plan_set_tuning(self._h, int(depth_chunks), int(view_batch)), "et_plan_set_tuning")
 
     def set_mu_check(self, check=True):
-        """True (default): osem keys its caches on a device fingerprint of the
-        attenuation map's content.  False: on its address only; call
-        invalidate() after changing the map in place."""
+        """True: osem keys its caches on a device fingerprint of the attenuation
+        map's content (mu_key "content").  False: on its address only (the
+        library's view under mu_key "version"); call invalidate() after changing
+        the map behind torch's back."""
         _lib.check(self._L.et_plan_set_mu_check(self._h, int(bool(check))), "et_plan_set_mu_check")
+        self.mu_key = "content" if check else "address"
 
     def invalidate(self):
         """Drop the cached OSEM data."""
@@ -441,7 +482,11 @@ class Batch:
         mus = None if attenuations is None else P(*[(p._arg(m, p.shape, "attenuation %d" % i).value
                                                      if m is not None else None)
                                                     for i, (p, m) in enumerate(zip(self.plans, attenuations))])
+        for i, p in enumerate(self.plans):
+            p._track_mu(None if attenuations is None else attenuations[i])
         st = self._L.et_batch_osem(self._h, acts, sinos, mus, int(subsets), int(iterations), float(epsilon),
                                    int(bool(use_graph)), Plan._stream())
         _lib.check(st, "et_batch_osem")
+        for a in activities:
+            Plan._written(a)
         return activities

@@ -3,9 +3,10 @@
     python -m niftyrec_amd.build          # or: python niftyrec_amd/build.py
 
 Every HIP source is compiled with packed-FP32 instruction selection disabled
-(-Xclang -target-feature -Xclang -packed-fp32-ops); the host layer keeps the
-reference's single-precision rotation arithmetic by compiling without
-floating-point contraction.
+(-Xclang -target-feature -Xclang -packed-fp32-ops); the kernels contract
+multiply-adds except where a pragma forbids it (-ffp-contract=fast-honor-pragmas),
+and the host layer keeps the reference's single-precision rotation arithmetic
+by compiling without floating-point contraction.
 """
 import os
 import subprocess
@@ -24,10 +25,15 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + os.path.join(ROOT, "
           "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument"]
 RMAX = 10  # largest PSF radius of the sweep kernels (ET_RMAX in et_internal.h)
+# Kernels contract multiply-adds into FMAs except where a function body says
+# `#pragma clang fp contract(off)` (sample positions, TwoSum): plain
+# -ffp-contract=fast lets the AMDGPU backend fuse those as well, which moves a
+# sample position by an ulp against the reference's rounding.
+KERNEL_FP = ["-ffp-contract=fast-honor-pragmas"]
 # (source, object name, extra flags); the sweep launchers are instantiated once
 # per PSF radius in their own translation unit so that hipcc runs in parallel
-SOURCES = ([("et_kernels.hip", "et_kernels", ["-ffp-contract=fast"])] +
-           [("et_sweep_r.hip", "et_sweep_r%d" % r, ["-ffp-contract=fast", "-DET_SWEEP_R=%d" % r])
+SOURCES = ([("et_kernels.hip", "et_kernels", KERNEL_FP)] +
+           [("et_sweep_r.hip", "et_sweep_r%d" % r, KERNEL_FP + ["-DET_SWEEP_R=%d" % r])
             for r in range(RMAX, -1, -1)] +
            [("et_host.cpp", "et_host", ["-x", "hip", "-ffp-contract=off"])])
 

@@ -177,10 +177,11 @@ def test_plan_profile_sampling():
     plan = _plan(act.shape, cams, psf)
     y = _t(orc.project(act, mu, cams, psf))
     X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
-    plan.osem(X, y, _t(mu), subsets=4, iterations=1)     # caches built outside the count
+    M = _t(mu)
+    plan.osem(X, y, M, subsets=4, iterations=1)     # caches built outside the count
     for every, want in ((1, 12), (3, 4), (5, 3)):
         plan.profile(True, categories=["project"], every=every)
-        plan.osem(X, y, _t(mu), subsets=4, iterations=3)  # 12 projection launches
+        plan.osem(X, y, M, subsets=4, iterations=3)  # 12 projection launches
         r = plan.profile_read()
         plan.profile(False)
         assert r["project"][0] == want and r["project"][1] > 0

@@ -273,3 +273,15 @@ def test_osem_cache_follows_the_attenuation_map():
     M2.mul_(0.25)
     plan.invalidate()
     report_parity("osem, address key + invalidate", run(M2), oracle(mu2 * np.float32(0.25)))
+    # content key (the C library's default): a map rewritten without torch
+    # knowing (through a second tensor on the same memory, whose version
+    # counter is its own) is still caught
+    pc = Plan(act.shape, cams, psf, mu_key="content")
+    Mc = torch.from_numpy(mu).cuda()
+    Xc = torch.ones(act.shape, device="cuda")
+    pc.osem(Xc, Y, Mc, subsets=3, iterations=1)
+    alias = Mc.detach()
+    alias.data.mul_(2.0)
+    Xc = torch.ones(act.shape, device="cuda")
+    pc.osem(Xc, Y, Mc, subsets=3, iterations=1)
+    report_parity("osem, content key, map rewritten behind torch", Xc.cpu().numpy(), oracle(mu * np.float32(2.0)))
