@@ -96,6 +96,8 @@ typedef ET_GLOBAL_AS const float g_cfloat;
 typedef ET_GLOBAL_AS float g_float;
 typedef ET_GLOBAL_AS const float4 g_cfloat4;
 typedef ET_GLOBAL_AS float4 g_float4;
+typedef ET_GLOBAL_AS const float2 g_cfloat2;
+typedef ET_GLOBAL_AS float2 g_float2;
 __device__ __forceinline__ float gload(const float *p) { return *(g_cfloat *)p; }
 __device__ __forceinline__ float4 gload4(const float *p) { return *(g_cfloat4 *)p; }
 __device__ __forceinline__ void gstore(float *p, float v) { *(g_float *)p = v; }

@@ -221,6 +221,18 @@ struct DevBuf {
 // Plan
 // ---------------------------------------------------------------------------
 
+// One grouped launch of the row-lane OSEM kernels: argument records and block
+// table at byte offsets of a device table buffer.
+struct YlLaunch {
+    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE
+    int R, RPL, NR, nthr;     // sweep template and block shape (pull: RPL)
+    size_t lds;               // dynamic LDS bytes (sweeps)
+    size_t args_off, tab_off;
+    int n, blocks;
+    double bytes;             // algorithmic bytes (the profiling scopes' figure)
+};
+enum { YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4 };
+
 struct et_plan {
     int device = 0;
     int nx = 0, ny = 0, nz = 0, V = 0;
@@ -241,6 +253,7 @@ struct et_plan {
     // its address when mu_check is off)
     DevBuf x_yf, sens, osem_recs, osem_meas, osem_E, osem_order;
     bool osem_valid = false;
+    bool osem_yl = false;           // the cached data are the row-lane kernels'
     int osem_S = 0, osem_D = 0, osem_Dp = 0, osem_Db = 0;
     bool osem_mu_yf = false;
     std::vector<int> osem_start;    // subset s = views osem_start[s] .. osem_start[s+1]-1 (subset order)
@@ -250,6 +263,13 @@ struct et_plan {
     unsigned long long osem_mu_key = 0;
     float osem_eps = 0.f;
     int mu_check = 1;               // 1: fingerprint mu's content on every OSEM call
+    // row-lane OSEM kernels (et_yl.h): fast-path plans with attenuation whose
+    // rotated plane fits the LDS of one block; rows = yu = 16 * yl_rpl
+    bool yl = false;
+    int yl_rpl = 0, yl_nr = 0, yl_w = 0, yl_nbuf = 0;  // rows per lane, column rounds, columns per round, LDS planes
+    size_t yl_lds = 0;                                 // LDS bytes of a sweep block
+    DevBuf osem_tables;                                // argument records and block tables of the OSEM launches
+    std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
     DevBuf fp_dev;                  // fingerprint (8 bytes)
     unsigned long long *fp_host = nullptr;  // pinned copy of the fingerprint
     // tuning
@@ -314,6 +334,31 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
     p->slabs = (ny + ET_T - 1) / ET_T;
     p->xp = nx + 2;
     p->zp = nz + 2;
+    // Row-lane OSEM kernels: rows = 16 * RPL >= ny with RPL in {2, 4, 6, 8},
+    // at most 3 rounds of up to 64 columns, and one or two LDS planes of
+    // (nx + 2R) x (rows + 4) floats within the device's per-block LDS.
+    p->yl = false;
+    if (p->fast) {
+        const int rows32 = ((ny + 31) / 32) * 32;
+        const int rpl = rows32 / 16, nr = (nx + 63) / 64;
+        if (rpl <= 8 && nr <= 3) {
+            const size_t plane = (size_t)(nx + 2 * p->R) * (rows32 + 4) * sizeof(float);
+            int dev = 0, lim = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+                lim = 0;
+            const int nbuf = p->R == 0 ? 1 : (2 * plane <= (size_t)lim ? 2 : (plane <= (size_t)lim ? 1 : 0));
+            if (nbuf > 0) {
+                p->yl = true;
+                p->yl_rpl = rpl;
+                p->yl_nr = nr;
+                p->yl_w = (((nx + nr - 1) / nr) + 3) / 4 * 4;
+                p->yl_nbuf = nbuf;
+                p->yl_lds = p->R == 0 ? 0 : nbuf * plane;
+            }
+        }
+    }
+    const int yrows = p->yl ? 16 * p->yl_rpl : p->slabs * ET_T;  // axial rows held per column
     // YF layout.  Guard rows: the activity-like volumes are gathered in windows
     // of rows y0 - R4 .. y0 + TS + R4 - 1 (R4 = R rounded up to 4); with R4
     // guard rows every 16-row slab's window starts on a 64-byte segment
@@ -327,7 +372,7 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
     // gathers' neighbouring columns on fewer memory channels (the projector ran
     // 12% slower at 160 floats than at 144 or 176).  Plans off the fast path
     // use 16 guard rows and tail rows throughout.
-    const int rows = p->slabs * ET_T;
+    const int rows = yrows;
     p->gmu = ET_GY;
     if (p->fast) {
         const int r4 = (p->R + 3) & ~3;
@@ -337,7 +382,7 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
         p->gy = ET_GY;
         p->yp = odd_segments(ET_GY + rows + ET_GY);
     }
-    p->yu = p->slabs * ET_T;
+    p->yu = rows;
     p->yf_elems = (size_t)p->yp * p->xp * p->zp;
     p->ustride = (size_t)p->yu * p->xp * p->zp;
     p->P = (size_t)nx * ny;
@@ -401,7 +446,7 @@ struct ProfScope {
     hipEvent_t a = nullptr, b = nullptr;
     ProfScope(et_plan *p_, int cat_, double bytes_) : p(p_), cat(cat_), bytes(bytes_)
     {
-        if (!((p->prof >> cat) & 1u)) return;
+        if (!p || !((p->prof >> cat) & 1u)) return;
         if (p->prof_seen[cat]++ % p->prof_every != 0) return;
         a = pool_event(p);
         b = pool_event(p);
@@ -884,6 +929,32 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
     }
 }
 
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct TableBytes {
+    std::vector<char> bytes;
+    template <class T>
+    size_t put(const std::vector<T> &v)
+    {
+        const size_t off = align256(bytes.size());
+        bytes.resize(off + v.size() * sizeof(T));
+        std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+        return off;
+    }
+};
+
+// Block table of a grouped launch: {first block, gx, gy, 0} per problem, each
+// range padded to a multiple of 8 blocks.
+struct BlockTable {
+    std::vector<int> t;
+    int total = 0;
+    void add(int gx, int gy)
+    {
+        t.insert(t.end(), {total, gx, gy, 0});
+        total += ((gx * gy + 7) / 8) * 8;
+    }
+};
+
 // MLEM update restricted to views [v0, v0+nv) of the plan with every volume in
 // the YF layout; the reference update of _et_array_interface.cpp:1080-1131.
 // sens_yf must hold the clamped sensitivity of these views.
@@ -1056,10 +1127,302 @@ unsigned long long mu_key_read(const et_plan *p, const float *mu)
     return *p->fp_host;
 }
 
+
+// ---- row-lane OSEM path (et_yl.h) ------------------------------------------------
+
+// Depth chunking of a row-lane sweep: the requested count rounded so that every
+// chunk holds at least one plane.
+void yl_chunks(const et_plan *p, int D, int &Deff, int &zlen)
+{
+    D = std::max(1, std::min(D, p->nz));
+    zlen = (p->nz + D - 1) / D;
+    Deff = (p->nz + zlen - 1) / zlen;
+}
+
+size_t yl_att_elems(const et_plan *p, int nv) { return (size_t)nv * p->nz * p->nx * p->yu; }
+size_t yl_plane_elems(const et_plan *p) { return (size_t)p->nx * p->yu; }  // [x'][rows]
+
+// Arguments of a row-lane sweep over nv views (records recs, factors att).
+EtSweepArgs yl_sweep_args(const et_plan *p, const float *recs, int nv, int D, const float *att)
+{
+    EtSweepArgs a = base_args(const_cast<et_plan *>(p));
+    int Deff, zlen;
+    yl_chunks(p, D, Deff, zlen);
+    a.views = recs;
+    a.V = nv;
+    a.D = Deff;
+    a.zlen = zlen;
+    a.att = const_cast<float *>(att);
+    a.yl_nbuf = p->yl_nbuf;
+    return a;
+}
+
+// Grouped launches of row-lane OSEM work over several plans; the block shapes
+// and templates come from each plan.  Sweeps share a launch when they share
+// (R, RPL, NR, block size); pulls when they share RPL.
+struct YlTable {
+    TableBytes tb;
+    std::vector<YlLaunch> L;
+
+    template <class Args, class Key, class Fill>
+    void grouped(int kind, const std::vector<et_plan *> &plans, Key key, Fill fill)
+    {
+        std::map<std::tuple<int, int, int, int>, std::vector<int>> groups;
+        for (size_t i = 0; i < plans.size(); i++) groups[key(plans[i])].push_back((int)i);
+        for (auto &g : groups) {
+            std::vector<Args> args;
+            BlockTable bt;
+            size_t lds = 0;
+            double bytes = 0;
+            for (int i : g.second) {
+                int gx = 0, gy = 0;
+                args.push_back(fill(i, gx, gy, bytes));
+                bt.add(gx, gy);
+                lds = std::max(lds, plans[i]->yl_lds);
+            }
+            const auto &k = g.first;
+            L.push_back({kind, std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k), lds, tb.put(args),
+                         tb.put(bt.t), (int)args.size(), bt.total, bytes});
+        }
+    }
+
+    static std::tuple<int, int, int, int> sweep_key(const et_plan *p)
+    {
+        return std::make_tuple(p->R, p->yl_rpl, p->yl_nr, 16 * p->yl_w);
+    }
+    static std::tuple<int, int, int, int> pull_key(const et_plan *p) { return std::make_tuple(0, p->yl_rpl, 0, 256); }
+
+    // Projection of each plan's views [v0, v0 + nv) of its OSEM records into its
+    // partial sums (depth chunks Dp of the plan).
+    void project(const std::vector<et_plan *> &plans, const std::vector<int> &v0, const std::vector<int> &nv,
+                 const std::vector<int> &Dp)
+    {
+        grouped<EtSweepArgs>(YL_PROJECT, plans, sweep_key, [&](int i, int &gx, int &gy, double &bytes) {
+            et_plan *p = plans[i];
+            bytes += (double)nv[i] * (8.0 * p->nvox + 4.0 * p->P);  // activity and mu (factors) per view, sinogram
+            EtSweepArgs a = yl_sweep_args(p, p->osem_recs.p + (size_t)v0[i] * ET_VIEW_REC, nv[i], Dp[i],
+                                          p->osem_E.p + (size_t)v0[i] * p->nz * yl_plane_elems(p));
+            a.act = p->x_yf.p;
+            a.part = p->part.p;
+            gx = a.D;
+            gy = nv[i];
+            return a;
+        });
+    }
+
+    void ratio(const std::vector<et_plan *> &plans, const std::vector<int> &v0, const std::vector<int> &nv,
+               const std::vector<int> &Dp)
+    {
+        std::vector<EtRatioArgs> ra;
+        BlockTable bt;
+        double bytes = 0;
+        for (size_t i = 0; i < plans.size(); i++) {
+            et_plan *p = plans[i];
+            int Deff, zlen;
+            yl_chunks(p, Dp[i], Deff, zlen);
+            const long long VP = (long long)nv[i] * (long long)yl_plane_elems(p);
+            ra.push_back({p->osem_meas.p + (size_t)v0[i] * yl_plane_elems(p), p->part.p, Deff, VP, p->osem_cur_eps,
+                          p->ratio.p});
+            bt.add((int)((VP + 255) / 256), 1);
+            bytes += (Deff + 2.0) * 4.0 * nv[i] * p->P;
+        }
+        L.push_back({YL_RATIO, 0, 0, 0, 256, 0, tb.put(ra), tb.put(bt.t), (int)ra.size(), bt.total, bytes});
+    }
+
+    // Rotated-frame backprojection of the ratio (ones = false) or of ones.
+    void bpu(const std::vector<et_plan *> &plans, const std::vector<int> &v0, const std::vector<int> &nv,
+             const std::vector<int> &Db, bool ones)
+    {
+        grouped<EtSweepArgs>(YL_BPU, plans, sweep_key, [&](int i, int &gx, int &gy, double &bytes) {
+            et_plan *p = plans[i];
+            bytes += (double)nv[i] * (8.0 * p->nvox + 4.0 * p->P);  // factors read, u write, sinogram read
+            EtSweepArgs a = yl_sweep_args(p, p->osem_recs.p + (size_t)v0[i] * ET_VIEW_REC, nv[i], Db[i],
+                                          p->osem_E.p + (size_t)v0[i] * p->nz * yl_plane_elems(p));
+            a.src = ones ? ET_SRC_ONES : ET_SRC_SINO;
+            a.sino = p->ratio.p;
+            a.u = p->u.p;
+            gx = a.D;
+            gy = nv[i];
+            return a;
+        });
+    }
+
+    // Pull of the rotated-frame volumes with the OSEM update (sens_out null) or
+    // the truncated store of the backprojection into sens_out[i].
+    void pull(const std::vector<et_plan *> &plans, const std::vector<int> &v0, const std::vector<int> &nv,
+              const std::vector<const float *> &sens, const std::vector<float *> *sens_out)
+    {
+        grouped<EtPullArgs>(sens_out ? YL_PULL_STORE : YL_PULL_OSEM, plans, pull_key,
+                            [&](int i, int &gx, int &gy, double &bytes) {
+            et_plan *p = plans[i];
+            bytes += (double)nv[i] * 4.0 * p->nvox + (sens_out ? 8.0 : 16.0) * p->nvox;
+            const float *r = p->osem_recs.p + (size_t)v0[i] * ET_VIEW_REC;
+            EtPullArgs a = sens_out ? pull_args(p, r, nv[i], p->u.p, false, ET_EPI_YF_STORE, false, true,
+                                                (*sens_out)[i], nullptr, nullptr)
+                                    : pull_args(p, r, nv[i], p->u.p, false, ET_EPI_OSEM, false, false, nullptr,
+                                                p->x_yf.p, sens[i]);
+            gx = (int)(((size_t)p->nx * p->nz + 15) / 16);
+            gy = 1;
+            return a;
+        });
+    }
+};
+
+// Issues launches [from, to) of a table uploaded to `dev`.
+void yl_issue(const float *dev, const std::vector<YlLaunch> &L, size_t from, size_t to, hipStream_t st,
+              et_plan *prof = nullptr)
+{
+    const char *base = reinterpret_cast<const char *>(dev);
+    for (size_t k = from; k < to; k++) {
+        const YlLaunch &l = L[k];
+        const void *args = base + l.args_off, *tab = base + l.tab_off;
+        const et_stream_t s = (et_stream_t)st;
+        switch (l.kind) {
+        case YL_PROJECT:
+        case YL_BPU: {
+            ProfScope ps(prof, l.kind == YL_PROJECT ? ET_PROF_PROJECT : ET_PROF_BPU, l.bytes);
+            check_launch(et_launch_yl_sweep(l.kind == YL_PROJECT ? 0 : 1, l.R, l.RPL, l.NR, l.nthr, l.lds,
+                                            static_cast<const EtSweepArgs *>(args), tab, l.n, l.blocks, s));
+            break;
+        }
+        case YL_RATIO: {
+            ProfScope ps(prof, ET_PROF_OTHER, l.bytes);
+            check_launch(et_launch_ratio_group(static_cast<const EtRatioArgs *>(args), tab, l.n, l.blocks, s));
+            break;
+        }
+        default: {
+            ProfScope ps(prof, ET_PROF_PULL, l.bytes);
+            check_launch(et_launch_yl_pull(l.kind == YL_PULL_OSEM ? ET_EPI_OSEM : ET_EPI_YF_STORE, l.RPL,
+                                           static_cast<const EtPullArgs *>(args), tab, l.n, l.blocks, s));
+        }
+        }
+    }
+}
+
+// Uploads a table into buf (synchronously with respect to the stream).
+void yl_upload(DevBuf &buf, const YlTable &t, hipStream_t st)
+{
+    buf.ensure((align256(t.tb.bytes.size()) + 3) / 4, false);
+    check_hip(hipMemcpyAsync(buf.p, t.tb.bytes.data(), t.tb.bytes.size(), hipMemcpyHostToDevice, st),
+              niftyrec_error_transfergpu);
+    check_hip(hipStreamSynchronize(st), niftyrec_error_transfergpu);
+}
+
+// Depth chunking of the row-lane projection (mode ET_MODE_PROJECT, timed with
+// the ratio that reads its partial sums) or backprojection (ET_MODE_BPU, timed
+// behind the chosen projection and ratio) on the first subset: candidates 8,
+// 16, 32 chunks and the count that puts one block per CU, three launches each
+// after a warm-up, in three interleaved rounds; the fastest round decides.
+int yl_tune_chunks(et_plan *p, int mode, int nv, int Dp = 0)
+{
+    int cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    std::vector<int> cand;
+    for (int D : {8, 16, 32, cus / std::max(nv, 1)}) {
+        int Deff, zlen;
+        yl_chunks(p, D, Deff, zlen);
+        if (Deff >= 1 && std::find(cand.begin(), cand.end(), Deff) == cand.end()) cand.push_back(Deff);
+    }
+    const std::vector<et_plan *> one{p};
+    const std::vector<int> v0{0}, nvv{nv};
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), niftyrec_error_kernel);
+    check_hip(hipEventCreate(&e1), niftyrec_error_kernel);
+    const unsigned prof = p->prof;
+    p->prof = 0;
+    std::vector<float> best(cand.size(), 1e30f);
+    DevBuf tables;
+    for (int round = 0; round < 3; round++) {
+        for (size_t ci = 0; ci < cand.size(); ci++) {
+            const int D = cand[ci];
+            YlTable t;
+            const std::vector<int> dp{mode == ET_MODE_PROJECT ? D : Dp}, db{D};
+            t.project(one, v0, nvv, dp);
+            t.ratio(one, v0, nvv, dp);
+            if (mode == ET_MODE_BPU) t.bpu(one, v0, nvv, db, false);
+            yl_upload(tables, t, p->st);
+            auto run = [&]() { yl_issue(tables.p, t.L, 0, t.L.size(), p->st); };
+            run();
+            check_hip(hipEventRecord(e0, p->st), niftyrec_error_kernel);
+            for (int r = 0; r < 3; r++) run();
+            check_hip(hipEventRecord(e1, p->st), niftyrec_error_kernel);
+            check_hip(hipEventSynchronize(e1), niftyrec_error_kernel);
+            float ms = 0.f;
+            check_hip(hipEventElapsedTime(&ms, e0, e1), niftyrec_error_kernel);
+            best[ci] = std::min(best[ci], ms);
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->prof = prof;
+    return cand[std::min_element(best.begin(), best.end()) - best.begin()];
+}
+
+// Row-lane caches of an OSEM rebuild: attenuation factors of every view (one
+// launch), chunkings, subset sensitivities, and the launch table of one
+// iteration.  The measured planes and the iterate are set by the caller.
+void yl_osem_build(et_plan *p, const float *mu_yf, int S, float eps, const std::vector<int> &start)
+{
+    const int maxsub = (p->V + S - 1) / S;
+    // every buffer the launch tables refer to is allocated before they are built
+    p->x_yf.ensure(p->yf_elems, true, p->st);
+    p->osem_meas.ensure((size_t)p->V * yl_plane_elems(p), true, p->st);
+    p->osem_E.ensure(yl_att_elems(p, p->V), false);
+    {
+        EtSweepArgs a = yl_sweep_args(p, p->osem_recs.p, p->V, 1, p->osem_E.p);
+        a.mu = mu_yf;
+        ProfScope ps(p, ET_PROF_OTHER, (double)p->V * 8.0 * p->nvox);
+        check_launch(et_launch_yl_factors(p->yl_rpl, a, (et_stream_t)p->st));
+    }
+    p->u.ensure((size_t)maxsub * p->ustride, true, p->st);
+    p->ratio.ensure((size_t)maxsub * yl_plane_elems(p), true, p->st);
+    p->part.ensure((size_t)p->nz * maxsub * yl_plane_elems(p), false);
+    p->osem_cur_eps = eps;
+    int Dp = p->tune_D, Db = p->tune_D;
+    if (p->tune_D <= 0) {
+        Dp = yl_tune_chunks(p, ET_MODE_PROJECT, start[1] - start[0]);
+        Db = yl_tune_chunks(p, ET_MODE_BPU, start[1] - start[0], Dp);
+    }
+    int zlen;
+    yl_chunks(p, Dp, p->osem_Dp, zlen);
+    yl_chunks(p, Db, p->osem_Db, zlen);
+    // sensitivities: backprojection of ones, truncated, clamped below at eps
+    p->sens.ensure((size_t)S * p->yf_elems, true, p->st);
+    const std::vector<et_plan *> one{p};
+    for (int s = 0; s < S; s++) {
+        YlTable t;
+        const std::vector<int> v0{start[s]}, nv{start[s + 1] - start[s]}, db{p->osem_Db};
+        std::vector<float *> out{p->sens.p + (size_t)s * p->yf_elems};
+        t.bpu(one, v0, nv, db, true);
+        t.pull(one, v0, nv, {}, &out);
+        yl_upload(p->osem_tables, t, p->st);
+        yl_issue(p->osem_tables.p, t.L, 0, t.L.size(), p->st, p);
+        ProfScope ps(p, ET_PROF_OTHER, 0);
+        check_launch(et_launch_yf_clamp_min(out[0], p->nx, p->ny, p->nz, p->yp, p->gy, eps, (et_stream_t)p->st));
+    }
+    // launch table of one iteration: per subset projection, ratio, backprojection, pull
+    YlTable t;
+    for (int s = 0; s < S; s++) {
+        const std::vector<int> v0{start[s]}, nv{start[s + 1] - start[s]}, dp{p->osem_Dp}, db{p->osem_Db};
+        const std::vector<const float *> sens{p->sens.p + (size_t)s * p->yf_elems};
+        t.project(one, v0, nv, dp);
+        t.ratio(one, v0, nv, dp);
+        t.bpu(one, v0, nv, db, false);
+        t.pull(one, v0, nv, sens, nullptr);
+    }
+    yl_upload(p->osem_tables, t, p->st);
+    p->yl_launches = t.L;
+}
+
 void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int S, float eps,
-                  unsigned long long mu_key)
+                  unsigned long long mu_key, bool allow_yl = true)
 {
     if (S <= 0 || S > p->V || !p->fast) throw EtError{niftyrec_error_parameters};
+    // the row-lane kernels need the attenuation factors (a map) and a plan whose
+    // plane fits their block
+    const bool use_yl = allow_yl && p->yl && mu != nullptr;
     // subset s: views s, s+S, ...; records and measured planes reordered by subset
     std::vector<int> order, start(S + 1, 0);
     for (int s = 0; s < S; s++) {
@@ -1070,7 +1433,7 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
     const int maxsub = (p->V + S - 1) / S;
     const int D = choose_D(p, maxsub);
     const bool rebuild = !p->osem_valid || p->osem_S != S || p->osem_D != D || p->osem_has_mu != (mu != nullptr) ||
-                         p->osem_mu_key != mu_key || p->osem_eps != eps;
+                         p->osem_mu_key != mu_key || p->osem_eps != eps || p->osem_yl != use_yl;
     // mu in the YF layout is read only to build the caches (the subset loop
     // reads the cached factors), so it is packed only for a rebuild
     if (mu && rebuild) pack(p, mu, p->mu_yf);
@@ -1088,8 +1451,36 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
         check_hip(hipMemcpyAsync(p->osem_order.p, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice,
                                  p->st),
                   niftyrec_error_transfergpu);
-        // chunking: fixed by et_plan_set_tuning, else timed on the first subset
         p->osem_valid = false;  // until the rebuild completes
+        if (use_yl) {
+            p->osem_mu_yf = true;
+            yl_osem_build(p, mu_yf, S, eps, start);
+            check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
+            p->osem_S = S;
+            p->osem_D = D;
+            p->osem_has_mu = true;
+            p->osem_mu_key = mu_key;
+            p->osem_eps = eps;
+            p->osem_yl = true;
+            p->osem_valid = true;
+        }
+    }
+    if (use_yl) {
+        // measured data in subset order, transposed to [v][x][rows]
+        p->osem_meas.ensure((size_t)p->V * yl_plane_elems(p), false);
+        {
+            ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->V * p->P);
+            check_launch(et_launch_gather_planes_t(p->osem_meas.p, meas, reinterpret_cast<const int *>(p->osem_order.p),
+                                                   p->V, p->nx, p->ny, p->yu, (et_stream_t)p->st));
+        }
+        pack(p, x, p->x_yf);
+        p->osem_start = start;
+        p->osem_cur_mu = mu_yf;
+        p->osem_cur_eps = eps;
+        return;
+    }
+    if (rebuild) {
+        // chunking: fixed by et_plan_set_tuning, else timed on the first subset
         int Dp = D, Db = D;
         if (p->tune_D <= 0) {
             Dp = tune_depth_chunks(p, ET_MODE_PROJECT, p->osem_recs.p, start[1], mu_yf);
@@ -1124,6 +1515,7 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
         p->osem_has_mu = mu != nullptr;
         p->osem_mu_key = mu_key;
         p->osem_eps = eps;
+        p->osem_yl = false;
         p->osem_valid = true;
     }
     // measured data in subset order
@@ -1146,6 +1538,10 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
 // Device work of subset s (views start[s] .. start[s+1]-1 in subset order).
 void osem_subset(et_plan *p, int s)
 {
+    if (p->osem_yl) {
+        yl_issue(p->osem_tables.p, p->yl_launches, 4 * (size_t)s, 4 * (size_t)s + 4, p->st, p);
+        return;
+    }
     const std::vector<int> &start = p->osem_start;
     const int nv = start[s + 1] - start[s];
     const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
@@ -1505,6 +1901,8 @@ struct et_batch {
         int n, blocks;
     };
     std::vector<Launch> launches;     // one OSEM iteration, in order
+    bool yl = false;                  // the iteration runs the row-lane kernels (yl_launches)
+    std::vector<YlLaunch> yl_launches;
     std::vector<const void *> key;    // buffers and chunkings the launch list refers to
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1527,32 +1925,6 @@ struct et_batch {
 
 namespace {
 
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-struct TableBytes {
-    std::vector<char> bytes;
-    template <class T>
-    size_t put(const std::vector<T> &v)
-    {
-        const size_t off = align256(bytes.size());
-        bytes.resize(off + v.size() * sizeof(T));
-        std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
-        return off;
-    }
-};
-
-// Block table of a grouped launch: {first block, gx, gy, 0} per problem, each
-// range padded to a multiple of 8 blocks.
-struct BlockTable {
-    std::vector<int> t;
-    int total = 0;
-    void add(int gx, int gy)
-    {
-        t.insert(t.end(), {total, gx, gy, 0});
-        total += ((gx * gy + 7) / 8) * 8;
-    }
-};
-
 std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
 {
     std::vector<const void *> k;
@@ -1564,6 +1936,7 @@ std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
         k.insert(k.end(), {p->x_yf.p, p->osem_meas.p, p->part.p, p->ratio.p, p->u.p, p->sens.p, p->osem_recs.p,
                            p->osem_E.p, p->osem_cur_mu, p->d_sep.p});
         k.push_back(reinterpret_cast<const void *>((uintptr_t)(p->osem_Dp * 1024 + p->osem_Db)));
+        k.push_back(reinterpret_cast<const void *>((uintptr_t)p->osem_yl));
     }
     return k;
 }
@@ -1573,6 +1946,35 @@ std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
 // multiplicative update.  The same argument records as em_subset_update.
 void batch_build(et_batch *b, int S, hipStream_t st)
 {
+    b->yl = !b->plans.empty();
+    for (const et_plan *p : b->plans) b->yl = b->yl && p->osem_yl;
+    if (b->yl) {
+        // row-lane kernels: per subset grouped projections, one ratio launch,
+        // grouped backprojections, grouped pulls with the update
+        YlTable t;
+        const size_t n = b->plans.size();
+        for (int s = 0; s < S; s++) {
+            std::vector<int> v0(n), nv(n), dp(n), db(n);
+            std::vector<const float *> sens(n);
+            for (size_t i = 0; i < n; i++) {
+                const et_plan *p = b->plans[i];
+                v0[i] = p->osem_start[s];
+                nv[i] = p->osem_start[s + 1] - v0[i];
+                dp[i] = p->osem_Dp;
+                db[i] = p->osem_Db;
+                sens[i] = p->sens.p + (size_t)s * p->yf_elems;
+            }
+            t.project(b->plans, v0, nv, dp);
+            t.ratio(b->plans, v0, nv, dp);
+            t.bpu(b->plans, v0, nv, db, false);
+            t.pull(b->plans, v0, nv, sens, nullptr);
+        }
+        b->drop_graph();
+        yl_upload(b->tables, t, st);
+        b->yl_launches = t.L;
+        b->launches.clear();
+        return;
+    }
     std::map<std::tuple<int, int, int>, std::vector<int>> groups;
     for (size_t i = 0; i < b->plans.size(); i++) {
         const et_plan *p = b->plans[i];
@@ -1639,6 +2041,10 @@ void batch_build(et_batch *b, int S, hipStream_t st)
 
 void batch_issue(const et_batch *b, hipStream_t st)
 {
+    if (b->yl) {
+        yl_issue(b->tables.p, b->yl_launches, 0, b->yl_launches.size(), st);
+        return;
+    }
     const char *base = reinterpret_cast<const char *>(b->tables.p);
     for (const auto &l : b->launches) {
         const void *args = base + l.args_off, *tab = base + l.tab_off;
@@ -1672,13 +2078,16 @@ void op_batch_osem(et_batch *b, float *const *act, const float *const *sino, con
         any_fp = any_fp || (m && p->mu_check);
     }
     if (any_fp) check_hip(hipStreamSynchronize(st), niftyrec_error_kernel);
+    // the row-lane kernels when every plan can take them, else the sweep kernels for all
+    bool all_yl = true;
+    for (size_t i = 0; i < b->plans.size(); i++) all_yl = all_yl && b->plans[i]->yl && mu && mu[i];
     for (size_t i = 0; i < b->plans.size(); i++) {
         et_plan *p = b->plans[i];
         const float *m = mu ? mu[i] : nullptr;
-        osem_prepare(p, act[i], sino[i], m, S, eps, mu_key_read(p, m));
+        osem_prepare(p, act[i], sino[i], m, S, eps, mu_key_read(p, m), all_yl);
     }
     const std::vector<const void *> key = batch_key(b, S, eps);
-    if (key != b->key || b->launches.empty()) {
+    if (key != b->key || (b->launches.empty() && b->yl_launches.empty())) {
         batch_build(b, S, st);
         b->key = key;
     }

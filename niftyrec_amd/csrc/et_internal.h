@@ -75,6 +75,8 @@ struct EtSweepArgs {
     float *u;                // U layout outputs
     float *att;              // attenuation factors e^{-A(x',y,z')} of the views, [V][nz][yu][nthr] with
                              // nthr = roundup(nx, 64) (the block width); see et_launch_sweep's has_mu
+                             // (row-lane kernels: [V][nz][nx][yu])
+    int yl_nbuf;             // row-lane sweeps: LDS planes (2: alternate, one barrier per plane; 1: two)
 };
 
 enum EtEpilogue {
@@ -170,5 +172,19 @@ int et_launch_partial_line(const float *act, const float *mu, const float *bgimg
 // Content fingerprint of n floats into *out (8 bytes of device memory): the
 // key under which the OSEM caches remember the attenuation map.
 int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_stream_t stream);
+// Row-lane kernels of the OSEM subset loop (et_yl.h).  Grouped launches only
+// (tab as et_launch_sweep_group): sweeps kind 0 projection, 1 rotated-frame
+// backprojection, gx = depth chunks, gy = views, nthr = 16 * W threads, `lds`
+// bytes of LDS; pull gx = ceil(nx * nz / 16), gy = 1, 256 threads.  RPL
+// (rows per lane) in {2, 4, 6, 8}, NR (column rounds) in {1, 2, 3}.
+int et_launch_yl_sweep(int kind, int R, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args,
+                       const void *tab, int n, int blocks, et_stream_t stream);
+int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void *tab, int n, int blocks,
+                      et_stream_t stream);
+// Attenuation factors of a.V views into a.att (row-lane layout) from a.mu (YF).
+int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream);
+// dst[v][x][y] = src[order[v]][y][x] (y < ny), 0 for ny <= y < rows.
+int et_launch_gather_planes_t(float *dst, const float *src, const int *order, int V, int nx, int ny, int rows,
+                              et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).
 int et_sweep_slab_rows(int mode, int R);

@@ -646,6 +646,29 @@ __global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict_
     if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
 }
 
+// dst[v][x][y] = src[order[v]][y][x] for y < ny, 0 for ny <= y < rows: the
+// measured planes of the row-lane kernels, gathered in subset order and
+// transposed (through a 32 x 33 LDS tile).  grid (ceil(nx/32), ceil(rows/32), V).
+__global__ void __launch_bounds__(256) k_gather_planes_t(float *__restrict__ dst, const float *__restrict__ src,
+                                                         const int *__restrict__ order, int nx, int ny, int rows)
+{
+    __shared__ float tile[32][33];
+    const int v = blockIdx.z;
+    const int xb = blockIdx.x * 32, yb = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const float *sp = src + (size_t)order[v] * nx * ny;
+    for (int k = ty; k < 32; k += 8) {
+        const int x = xb + tx, y = yb + k;
+        tile[k][tx] = (x < nx && y < ny) ? sp[x + (size_t)nx * y] : 0.f;
+    }
+    __syncthreads();
+    float *dp = dst + (size_t)v * nx * rows;
+    for (int k = ty; k < 32; k += 8) {
+        const int y = yb + tx, x = xb + k;
+        if (x < nx && y < rows) dp[y + (size_t)rows * x] = tile[tx][k];
+    }
+}
+
 inline unsigned nblk(size_t n, unsigned b) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace
@@ -870,5 +893,65 @@ int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_
     const unsigned blocks = (unsigned)std::min<size_t>(nblk(n, 256), 2048);
     hipLaunchKernelGGL(k_fingerprint, dim3(std::max(1u, blocks)), dim3(256), 0, st, reinterpret_cast<const unsigned *>(p),
                        n, out);
+    return (int)hipGetLastError();
+}
+
+int et_launch_yl_sweep(int kind, int R, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args,
+                       const void *tab, int n, int blocks, et_stream_t stream)
+{
+    if (n <= 0 || blocks <= 0 || nthr <= 0 || nthr > 1024 || (nthr & 63)) return (int)hipErrorInvalidValue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int4 *t = reinterpret_cast<const int4 *>(tab);
+    switch (R) {
+#define ET_R_CASE(r) \
+    case r: return et_yl_launch_r##r(kind, RPL, NR, nthr, lds, args, t, n, blocks, st);
+        ET_R_CASE(0) ET_R_CASE(1) ET_R_CASE(2) ET_R_CASE(3) ET_R_CASE(4) ET_R_CASE(5)
+        ET_R_CASE(6) ET_R_CASE(7) ET_R_CASE(8) ET_R_CASE(9) ET_R_CASE(10)
+#undef ET_R_CASE
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void *tab, int n, int blocks,
+                      et_stream_t stream)
+{
+    if (n <= 0 || blocks <= 0) return (int)hipErrorInvalidValue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int4 *t = reinterpret_cast<const int4 *>(tab);
+#define ET_YLP(rpl)                                                                                          \
+    if (RPL == rpl) {                                                                                        \
+        if (epilogue == ET_EPI_OSEM)                                                                         \
+            hipLaunchKernelGGL((k_yl_pull<rpl, ET_EPI_OSEM>), dim3(blocks), dim3(256), 0, st, args, t, n);    \
+        else if (epilogue == ET_EPI_YF_STORE)                                                                \
+            hipLaunchKernelGGL((k_yl_pull<rpl, ET_EPI_YF_STORE>), dim3(blocks), dim3(256), 0, st, args, t, n); \
+        else                                                                                                 \
+            return (int)hipErrorInvalidValue;                                                                \
+        return (int)hipGetLastError();                                                                       \
+    }
+    ET_YLP(2) ET_YLP(4) ET_YLP(6) ET_YLP(8)
+#undef ET_YLP
+    return (int)hipErrorInvalidValue;
+}
+
+int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream)
+{
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const dim3 g((a.nx + 15) / 16, a.V);
+#define ET_YLF(rpl)                                                                   \
+    if (RPL == rpl) {                                                                 \
+        hipLaunchKernelGGL((k_yl_factors<rpl>), g, dim3(256), 0, st, a);              \
+        return (int)hipGetLastError();                                                \
+    }
+    ET_YLF(2) ET_YLF(4) ET_YLF(6) ET_YLF(8)
+#undef ET_YLF
+    return (int)hipErrorInvalidValue;
+}
+
+int et_launch_gather_planes_t(float *dst, const float *src, const int *order, int V, int nx, int ny, int rows,
+                              et_stream_t stream)
+{
+    dim3 g((nx + 31) / 32, (rows + 31) / 32, V);
+    hipLaunchKernelGGL(k_gather_planes_t, g, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), dst, src, order, nx,
+                       ny, rows);
     return (int)hipGetLastError();
 }

@@ -5,6 +5,7 @@
 // per radius with -DET_SWEEP_R=R) so that the library builds in parallel.
 #pragma once
 #include "et_device.h"
+#include "et_yl.h"
 
 namespace {
 
@@ -568,9 +569,38 @@ static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_
     }
 }
 
+// Row-lane sweeps (et_yl.h) of PSF radius R: kind 0 projection, 1 rotated-frame
+// backprojection; a grouped launch of `blocks` blocks of nthr = 16 * W threads
+// with `lds` bytes of dynamic LDS (two planes of the widest problem).
+template <int R>
+static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args, const int4 *tab,
+                       int n, int blocks, hipStream_t st)
+{
+#define ET_YL(rpl, nr)                                                                                            \
+    if (RPL == rpl && NR == nr) {                                                                                 \
+        if (kind == 0) {                                                                                          \
+            allow_lds(k_yl_project<R, rpl, nr>, lds);                                                             \
+            hipLaunchKernelGGL((k_yl_project<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);      \
+        } else {                                                                                                  \
+            allow_lds(k_yl_bpu<R, rpl, nr>, lds);                                                                 \
+            hipLaunchKernelGGL((k_yl_bpu<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);          \
+        }                                                                                                         \
+        return (int)hipGetLastError();                                                                            \
+    }
+    ET_YL(2, 1) ET_YL(2, 2) ET_YL(2, 3) ET_YL(4, 1) ET_YL(4, 2) ET_YL(4, 3)
+    ET_YL(6, 1) ET_YL(6, 2) ET_YL(6, 3) ET_YL(8, 1) ET_YL(8, 2) ET_YL(8, 3)
+#undef ET_YL
+    return (int)hipErrorInvalidValue;
+}
+
 // Per-radius launchers (et_sweep_r.hip): sweep of `mode` with PSF radius R,
-// one problem (grp == nullptr) or a grouped launch.
+// one problem (grp == nullptr) or a grouped launch; row-lane sweeps.
 #define ET_SWEEP_DECL(r) int et_sweep_launch_r##r(int mode, int has_mu, const EtSweepArgs &a, hipStream_t st, const GroupLaunch *grp);
 ET_SWEEP_DECL(0) ET_SWEEP_DECL(1) ET_SWEEP_DECL(2) ET_SWEEP_DECL(3) ET_SWEEP_DECL(4) ET_SWEEP_DECL(5)
 ET_SWEEP_DECL(6) ET_SWEEP_DECL(7) ET_SWEEP_DECL(8) ET_SWEEP_DECL(9) ET_SWEEP_DECL(10)
 #undef ET_SWEEP_DECL
+#define ET_YL_DECL(r) int et_yl_launch_r##r(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args, \
+                                            const int4 *tab, int n, int blocks, hipStream_t st);
+ET_YL_DECL(0) ET_YL_DECL(1) ET_YL_DECL(2) ET_YL_DECL(3) ET_YL_DECL(4) ET_YL_DECL(5)
+ET_YL_DECL(6) ET_YL_DECL(7) ET_YL_DECL(8) ET_YL_DECL(9) ET_YL_DECL(10)
+#undef ET_YL_DECL

@@ -13,3 +13,9 @@ int ET_SWEEP_CAT(et_sweep_launch_r, ET_SWEEP_R)(int mode, int has_mu, const EtSw
 {
     return launch_sweep_r<ET_SWEEP_R>(mode, has_mu, a, st, grp);
 }
+
+int ET_SWEEP_CAT(et_yl_launch_r, ET_SWEEP_R)(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args,
+                                              const int4 *tab, int n, int blocks, hipStream_t st)
+{
+    return launch_yl_r<ET_SWEEP_R>(kind, RPL, NR, nthr, lds, args, tab, n, blocks, st);
+}

@@ -1,0 +1,505 @@
+// et_yl.h -- row-lane kernels of the OSEM subset loop for gfx950 (CDNA4).
+//
+// Lane layout: threadIdx.x = 16 * slot + lane.  A 16-lane DPP row (one slot)
+// holds one rotated column (x', z') with its whole axial extent: lane l owns
+// the RPL consecutive rows RPL*l .. RPL*l + RPL - 1 of the plan's
+// rows = 16 * RPL >= ny (rows beyond ny are zero in every source).  An axial
+// rotation leaves y alone, so the bilinear sample of (x', z') is the same for
+// every lane of the row, and each of its four corner loads is one contiguous
+// run of rows floats of the YF layout: coalesced by construction.
+//
+// PSF: the axial (column) pass takes the R rows beyond a lane's run from its
+// DPP neighbours (row_shr / row_shl with bound control, which reads 0 past the
+// ends of the row: the zero border of the reference convolution); the x'
+// (row) pass goes through LDS.  A sweep block owns one (view, depth chunk) and
+// all nx columns of each plane (NR rounds of W = blockDim / 16 columns), so the
+// row pass needs no halo recomputation: the LDS plane keeps R zero columns on
+// either side.  With two LDS planes alternating between depth planes, one
+// block barrier per plane orders every write after the reads of the same
+// buffer (a wave reaches plane z + 2's write only after every wave has passed
+// plane z + 1's barrier, which follows its plane-z reads); with one plane
+// (a.yl_nbuf == 1) a second barrier follows the reads.
+//
+// Layouts (rows = 16 * RPL, y fastest everywhere):
+//   factors   att[v][z'][x'][rows]  e^{-A(x', y, z')}, A the inclusive sum of mu'
+//   partials  part[d][v][x'][rows]  projection of depth chunk d
+//   measured counts and ratio       [v][x'][rows]
+//   U         u + v*ustride + y + rows*((x'+1) + xp*(z'+1))  (zero border in x', z')
+// Activity, attenuation and sensitivity are in the YF layout of the plan, whose
+// columns hold at least `rows` rows.
+#pragma once
+#include "et_device.h"
+
+namespace {
+
+// Value held by lane (l - K) (dpp_shr) or (l + K) (dpp_shl) of the same 16-lane
+// row; 0 where that lane lies outside the row.
+template <int K>
+__device__ __forceinline__ float dpp_shr(float v)
+{
+    static_assert(K >= 1 && K <= 15, "row shift");
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 + K, 0xf, 0xf, true));
+}
+template <int K>
+__device__ __forceinline__ float dpp_shl(float v)
+{
+    static_assert(K >= 1 && K <= 15, "row shift");
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x100 + K, 0xf, 0xf, true));
+}
+
+// RPL consecutive floats (RPL even; 16-byte vectors when RPL % 4 == 0).
+template <int RPL>
+__device__ __forceinline__ void yl_load(const float *p, float *v)
+{
+    if constexpr (RPL % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RPL / 4; q++) {
+            const float4 t = gload4(p + 4 * q);
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL / 2; q++) {
+            const float2 t = *(g_cfloat2 *)(p + 2 * q);
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    }
+}
+
+template <int RPL>
+__device__ __forceinline__ void yl_store(float *p, const float *v)
+{
+    if constexpr (RPL % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RPL / 4; q++)
+            gstore4(p + 4 * q, make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]));
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL / 2; q++) *(g_float2 *)(p + 2 * q) = make_float2(v[2 * q], v[2 * q + 1]);
+    }
+}
+
+template <int RPL>
+__device__ __forceinline__ void lds_store(float *p, const float *v)
+{
+    if constexpr (RPL % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RPL / 4; q++)
+            *reinterpret_cast<float4 *>(p + 4 * q) = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL / 2; q++) *reinterpret_cast<float2 *>(p + 2 * q) = make_float2(v[2 * q], v[2 * q + 1]);
+    }
+}
+
+template <int RPL>
+__device__ __forceinline__ void lds_load(const float *p, float *v)
+{
+    if constexpr (RPL % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RPL / 4; q++) {
+            const float4 t = *reinterpret_cast<const float4 *>(p + 4 * q);
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL / 2; q++) {
+            const float2 t = *reinterpret_cast<const float2 *>(p + 2 * q);
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    }
+}
+
+// Bilinear sample of the lane's RPL rows; p00 points at the lane's first row of
+// corner (x0, z0), dx and dz are the column strides in x and z.  Same
+// arithmetic order as the sweep kernels' gathers: c00 w00 + c01 w01 + c10 w10
+// + c11 w11.
+template <int RPL>
+__device__ __forceinline__ void yl_bilinear(const float *p00, int dx, int dz, const Samp &s, float *out)
+{
+    float c[4][RPL];
+    yl_load<RPL>(p00, c[0]);
+    yl_load<RPL>(p00 + dx, c[1]);
+    yl_load<RPL>(p00 + dz, c[2]);
+    yl_load<RPL>(p00 + dz + dx, c[3]);
+#pragma unroll
+    for (int i = 0; i < RPL; i++) out[i] = c[0][i] * s.w00 + c[1][i] * s.w01 + c[2][i] * s.w10 + c[3][i] * s.w11;
+}
+
+// The R rows on either side of a lane's run, from its row neighbours:
+// ext[R - t] = row -t (lane l - ceil(t / RPL)), ext[R + RPL + t] = row RPL + t
+// (lane l + 1 + t / RPL).  The DPP controls are immediates, hence the recursion.
+template <int R, int RPL, int T>
+struct YlHalo {
+    __device__ static __forceinline__ void fill(const float *in, float *ext)
+    {
+        if constexpr (T < R) {
+            constexpr int tb = T + 1;
+            constexpr int kb = (tb + RPL - 1) / RPL;
+            ext[R - tb] = dpp_shr<kb>(in[kb * RPL - tb]);
+            ext[R + RPL + T] = dpp_shl<1 + T / RPL>(in[T % RPL]);
+            YlHalo<R, RPL, T + 1>::fill(in, ext);
+        }
+    }
+};
+
+// Axial PSF pass of the lane's rows:
+//   out[i] = sum_{j=0}^{2R} in[i + j - R] * kc[2R - j]
+// (the convolution of _et_convolveSeparable2D.cpp:28-39, in the order of the
+// sweep kernels' column_pass).  Every lane of the 16-lane row must be active.
+template <int R, int RPL>
+__device__ __forceinline__ void yl_column_pass(const float *in, const float *__restrict__ kc, float *out)
+{
+    if constexpr (R == 0) {
+#pragma unroll
+        for (int i = 0; i < RPL; i++) out[i] = in[i];
+    } else {
+        float ext[RPL + 2 * R];
+#pragma unroll
+        for (int t = 0; t < RPL; t++) ext[R + t] = in[t];
+        YlHalo<R, RPL, 0>::fill(in, ext);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            float t = 0.f;
+#pragma unroll
+            for (int j = 0; j <= 2 * R; j++) t = fmaf(ext[i + j], cload(kc + 2 * R - j), t);
+            out[i] = t;
+        }
+    }
+}
+
+// x' (row) pass of the lane's rows of column xc from an LDS plane whose column
+// c + R holds column c (R zero columns on either side), column stride CS:
+//   out[i] = sum_{j=0}^{2R} plane[xc + j - R][row i] * kr[2R - j]
+template <int R, int RPL>
+__device__ __forceinline__ void yl_row_pass(const float *plane, int CS, int xc, int row0,
+                                            const float *__restrict__ kr, float *out)
+{
+#pragma unroll
+    for (int i = 0; i < RPL; i++) out[i] = 0.f;
+#pragma unroll
+    for (int j = 0; j <= 2 * R; j++) {
+        const float kj = cload(kr + 2 * R - j);
+        float t[RPL];
+        lds_load<RPL>(plane + (xc + j) * CS + row0, t);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) out[i] = fmaf(t[i], kj, out[i]);
+    }
+}
+
+// Floats of one LDS plane of a row-lane sweep block (column stride rows + 4).
+__host__ __device__ constexpr int yl_plane_floats(int nx, int R, int rows) { return (nx + 2 * R) * (rows + 4); }
+
+// Zeroes both LDS planes of the block.
+__device__ __forceinline__ void yl_zero_lds(float *lds, int n)
+{
+    for (int i = 4 * (int)threadIdx.x; i < n; i += 4 * (int)blockDim.x)
+        *reinterpret_cast<float4 *>(lds + i) = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------------------
+// Projection with cached attenuation factors.  Block (d, v): depth chunk d of
+// view v, all columns.  part[d][v][x'][rows] = sum over the chunk's planes of
+// PSF_z'(act')(x', y, z') e^{-A(x', y, z')}, accumulated with Kahan's
+// correction (+ bg e^{-A_tot} in the last chunk).  Reference
+// emission-lib/_et.cpp:218-281 with _et_line_integral_attenuated.cpp:26-36.
+// ------------------------------------------------------------------------------
+template <int R, int RPL, int NR>
+__device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk &b)
+{
+    constexpr int K = 2 * R + 1;
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4, W = blockDim.x >> 4;
+    const int d = b.x, v = b.y;
+    const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
+    const int rows = a.yu, CS = rows + 4, row0 = RPL * lane;
+    const int dx = a.yp, dz = a.yp * a.xp;
+    const int PF = yl_plane_floats(a.nx, R, rows);
+    const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
+    if constexpr (R > 0) yl_zero_lds(lds, a.yl_nbuf * PF);
+    const float *act = a.act + a.gy + row0;
+
+    float acc[NR][RPL], cmp[NR][RPL];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            acc[r][i] = 0.f;
+            cmp[r][i] = 0.f;
+        }
+    auto kahan = [&](int r, int i, float h, float e) {
+        const float y = fmaf(h, e, -cmp[r][i]);
+        const float t = acc[r][i] + y;
+        cmp[r][i] = (t - acc[r][i]) - y;
+        acc[r][i] = t;
+    };
+
+    for (int z = zb; z < ze; z++) {
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
+        float h[NR][RPL];
+        if constexpr (R > 0) {
+            float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) {
+                    const Samp s = fast_sample(fv, (float)xc, z, a.nx, a.nz, dx, dz);
+                    float bl[RPL], cp[RPL];
+                    yl_bilinear<RPL>(act + s.off, dx, dz, s, bl);
+                    yl_column_pass<R, RPL>(bl, kz + K, cp);
+                    lds_store<RPL>(plane + (xc + R) * CS + row0, cp);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) yl_row_pass<R, RPL>(plane, CS, xc, row0, kz, h[r]);
+            }
+            if (a.yl_nbuf == 1) __syncthreads();  // one plane: the next writes wait for these reads
+        } else {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) {
+                    const Samp s = fast_sample(fv, (float)xc, z, a.nx, a.nz, dx, dz);
+                    yl_bilinear<RPL>(act + s.off, dx, dz, s, h[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = r * W + slot;
+            if (xc < a.nx) {
+                float e[RPL];
+                yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e);
+#pragma unroll
+                for (int i = 0; i < RPL; i++) kahan(r, i, h[r][i], e[i]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int xc = r * W + slot;
+        if (xc >= a.nx) continue;
+        if (d == a.D - 1 && a.bg_line != 0.f) {
+            // e^{-A} along the whole ray = the factor of the last plane
+            float e[RPL];
+            yl_load<RPL>(a.att + (((size_t)v * a.nz + a.nz - 1) * a.nx + xc) * rows + row0, e);
+#pragma unroll
+            for (int i = 0; i < RPL; i++) kahan(r, i, a.bg_line, e[i]);
+        }
+        float out[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; i++) out[i] = acc[r][i] - cmp[r][i];
+        yl_store<RPL>(a.part + (((size_t)d * a.V + v) * a.nx + xc) * rows + row0, out);
+    }
+}
+
+// ------------------------------------------------------------------------------
+// Rotated-frame backprojection with cached attenuation factors:
+// u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ), s the ratio [v][x'][rows]
+// (ET_SRC_SINO) or 1 on the rows below ny (ET_SRC_ONES).  Reference
+// emission-lib/_et.cpp:441-471 (line backprojection, then the PSF).
+// ------------------------------------------------------------------------------
+template <int R, int RPL, int NR>
+__device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
+{
+    constexpr int K = 2 * R + 1;
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4, W = blockDim.x >> 4;
+    const int d = b.x, v = b.y;
+    const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
+    const int rows = a.yu, CS = rows + 4, row0 = RPL * lane;
+    const int PF = yl_plane_floats(a.nx, R, rows);
+    if constexpr (R > 0) yl_zero_lds(lds, a.yl_nbuf * PF);
+
+    float sv[NR][RPL];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int xc = r * W + slot;
+        if (xc >= a.nx) continue;
+        if (a.src == ET_SRC_ONES) {
+#pragma unroll
+            for (int i = 0; i < RPL; i++) sv[r][i] = (row0 + i < a.ny) ? 1.f : 0.f;
+        } else {
+            yl_load<RPL>(a.sino + ((size_t)v * a.nx + xc) * rows + row0, sv[r]);
+        }
+    }
+    float *uv = a.u + (size_t)v * a.ustride + row0;
+    for (int z = zb; z < ze; z++) {
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
+        float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = r * W + slot;
+            if (xc < a.nx) {
+                float e[RPL], w[RPL];
+                yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e);
+#pragma unroll
+                for (int i = 0; i < RPL; i++) w[i] = sv[r][i] * e[i];
+                if constexpr (R > 0) {
+                    float cp[RPL];
+                    yl_column_pass<R, RPL>(w, kz + K, cp);
+                    lds_store<RPL>(plane + (xc + R) * CS + row0, cp);
+                } else {
+                    yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), w);
+                }
+            }
+        }
+        if constexpr (R > 0) {
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) {
+                    float o[RPL];
+                    yl_row_pass<R, RPL>(plane, CS, xc, row0, kz, o);
+                    yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), o);
+                }
+            }
+            if (a.yl_nbuf == 1) __syncthreads();
+        }
+    }
+}
+
+template <int R, int RPL, int NR>
+__global__ void __launch_bounds__(1024) k_yl_project(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
+                                                     int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
+    yl_project_body<R, RPL, NR>(a, b);
+}
+
+template <int R, int RPL, int NR>
+__global__ void __launch_bounds__(1024) k_yl_bpu(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
+                                                 int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
+    yl_bpu_body<R, RPL, NR>(a, b);
+}
+
+// ------------------------------------------------------------------------------
+// Attenuation factors of every view (no PSF): att[v][z'][x'][rows] =
+// e^{-A(x', y, z')}, A the inclusive sum along depth of the resampled mu,
+// accumulated as a compensated (hi, lo) pair.  Block: 16 columns of one view,
+// all planes.  grid (ceil(nx / 16), V), 256 threads.
+// ------------------------------------------------------------------------------
+template <int RPL>
+__device__ __forceinline__ void yl_factor_body(const EtSweepArgs &a)
+{
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int xc = blockIdx.x * 16 + slot, v = blockIdx.y;
+    if (xc >= a.nx) return;  // the whole 16-lane row
+    const int rows = a.yu, row0 = RPL * lane;
+    const int dx = a.yp, dz = a.yp * a.xp;
+    const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
+    const float *mu = a.mu + a.gmu + row0;
+    float A[RPL], Al[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; i++) {
+        A[i] = 0.f;
+        Al[i] = 0.f;
+    }
+    for (int z = 0; z < a.nz; z++) {
+        const Samp s = fast_sample(fv, (float)xc, z, a.nx, a.nz, dx, dz);
+        float m[RPL], e[RPL];
+        yl_bilinear<RPL>(mu + s.off, dx, dz, s, m);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            comp_add(A[i], Al[i], m[i]);
+            e[i] = exp_neg(A[i] + Al[i]);
+        }
+        yl_store<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e);
+    }
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(256) k_yl_factors(EtSweepArgs a)
+{
+    yl_factor_body<RPL>(a);
+}
+
+// ------------------------------------------------------------------------------
+// Pull of the rotated-frame volumes of a.V views into the image frame (the
+// reference inverse-rotation pull: negated angles in ZYX order, trilinear with
+// zero padding, views accumulated in order; emission-lib/_et.cpp:474-498), with
+// the OSEM update x *= max(b, 0) / sens (ET_EPI_OSEM) or the truncated store of
+// b (ET_EPI_YF_STORE) on the rows below ny.  Block: 16 image columns (x, z),
+// column blocks contiguous per XCD.  256 threads.
+// ------------------------------------------------------------------------------
+template <int RPL, int EPI>
+__device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
+{
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int cb = ((b.gx & 7) == 0) ? (b.lin & 7) * (b.gx >> 3) + (b.lin >> 3) : b.lin;
+    const int c = cb * 16 + slot;
+    if (c >= a.nx * a.nz) return;
+    const int x = c % a.nx, z = c / a.nx;
+    const int rows = a.yu, row0 = RPL * lane;
+    const int dx = rows, dz = rows * a.xp;
+    float acc[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; i++) acc[i] = 0.f;
+    for (int v = 0; v < a.V; v++) {
+        const float *rec = a.views + (size_t)v * ET_VIEW_REC;
+        const float qx = pos2(cload(rec + 12), (float)x, cload(rec + 14), (float)z, cload(rec + 15));
+        const float qz = pos2(cload(rec + 20), (float)x, cload(rec + 22), (float)z, cload(rec + 23));
+        const Samp s = make_samp(qx, qz, a.nx, a.nz, dx, dz);
+        float t[RPL];
+        yl_bilinear<RPL>(a.u + (size_t)v * a.ustride + row0 + s.off, dx, dz, s, t);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) acc[i] += t[i];
+    }
+    const size_t base = (a.gy + row0) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
+    float o[RPL];
+    if constexpr (EPI == ET_EPI_OSEM) {
+        float xv[RPL], sv[RPL];
+        yl_load<RPL>(a.x + base, xv);
+        yl_load<RPL>(a.sens + base, sv);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            const float bb = acc[i] < 0.f ? 0.f : acc[i];
+            o[i] = (row0 + i < a.ny) ? xv[i] * (bb / sv[i]) : xv[i];
+        }
+        yl_store<RPL>(a.x + base, o);
+    } else {
+        float ov[RPL];
+        yl_load<RPL>(a.out + base, ov);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            float val = a.accumulate ? ov[i] + acc[i] : acc[i];
+            if (a.truncate && val < 0.f) val = 0.f;
+            o[i] = (row0 + i < a.ny) ? val : ov[i];
+        }
+        yl_store<RPL>(a.out + base, o);
+    }
+}
+
+template <int RPL, int EPI>
+__global__ void __launch_bounds__(256) k_yl_pull(const EtPullArgs *__restrict__ args, const int4 *__restrict__ tab,
+                                                 int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtPullArgs a = ((ET_CONST_AS const EtPullArgs *)args)[j];
+    yl_pull_body<RPL, EPI>(a, b);
+}
+
+}  // namespace
