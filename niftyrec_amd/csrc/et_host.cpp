@@ -1900,9 +1900,11 @@ struct et_batch {
         size_t args_off, tab_off;     // byte offsets into `tables`
         int n, blocks;
     };
-    std::vector<Launch> launches;     // one OSEM iteration, in order
-    bool yl = false;                  // the iteration runs the row-lane kernels (yl_launches)
-    std::vector<YlLaunch> yl_launches;
+    std::vector<Launch> launches;     // sweep-kernel plans' launches of one OSEM iteration, in order
+    std::vector<YlLaunch> yl_launches;  // row-lane plans' launches (table yl_tables)
+    DevBuf yl_tables;
+    std::vector<size_t> subset_begin;   // subset s: launches [2s], yl_launches [2s+1] (index 2S, 2S+1: ends)
+    int S = 0;
     std::vector<const void *> key;    // buffers and chunkings the launch list refers to
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -1941,53 +1943,56 @@ std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
     return k;
 }
 
-// Launch list of one OSEM iteration: per subset, grouped projections, the
-// ratios, grouped rotated-frame backprojections, then the pulls with the
-// multiplicative update.  The same argument records as em_subset_update.
+// Launch lists of one OSEM iteration.  Plans whose cached data are the
+// row-lane kernels' take those (yl_launches, one table), the others the sweep
+// kernels (launches): per subset, grouped projections, the ratios, grouped
+// rotated-frame backprojections, then the pulls with the multiplicative
+// update, with the same argument records as a plan run alone, so a batch
+// result is bit-identical to its plans' own et_plan_osem.  The two lists are
+// interleaved subset by subset (subset_begin).
 void batch_build(et_batch *b, int S, hipStream_t st)
 {
-    b->yl = !b->plans.empty();
-    for (const et_plan *p : b->plans) b->yl = b->yl && p->osem_yl;
-    if (b->yl) {
-        // row-lane kernels: per subset grouped projections, one ratio launch,
-        // grouped backprojections, grouped pulls with the update
-        YlTable t;
-        const size_t n = b->plans.size();
-        for (int s = 0; s < S; s++) {
-            std::vector<int> v0(n), nv(n), dp(n), db(n);
-            std::vector<const float *> sens(n);
-            for (size_t i = 0; i < n; i++) {
-                const et_plan *p = b->plans[i];
-                v0[i] = p->osem_start[s];
-                nv[i] = p->osem_start[s + 1] - v0[i];
-                dp[i] = p->osem_Dp;
-                db[i] = p->osem_Db;
-                sens[i] = p->sens.p + (size_t)s * p->yf_elems;
-            }
-            t.project(b->plans, v0, nv, dp);
-            t.ratio(b->plans, v0, nv, dp);
-            t.bpu(b->plans, v0, nv, db, false);
-            t.pull(b->plans, v0, nv, sens, nullptr);
+    std::vector<et_plan *> ylp, oldp;
+    for (et_plan *p : b->plans) (p->osem_yl ? ylp : oldp).push_back(p);
+    b->subset_begin.assign((size_t)2 * (S + 1), 0);
+    // row-lane plans
+    YlTable t;
+    const size_t n = ylp.size();
+    for (int s = 0; s < S; s++) {
+        b->subset_begin[2 * s + 1] = t.L.size();
+        if (n == 0) continue;
+        std::vector<int> v0(n), nv(n), dp(n), db(n);
+        std::vector<const float *> sens(n);
+        for (size_t i = 0; i < n; i++) {
+            const et_plan *p = ylp[i];
+            v0[i] = p->osem_start[s];
+            nv[i] = p->osem_start[s + 1] - v0[i];
+            dp[i] = p->osem_Dp;
+            db[i] = p->osem_Db;
+            sens[i] = p->sens.p + (size_t)s * p->yf_elems;
         }
-        b->drop_graph();
-        yl_upload(b->tables, t, st);
-        b->yl_launches = t.L;
-        b->launches.clear();
-        return;
+        t.project(ylp, v0, nv, dp);
+        t.ratio(ylp, v0, nv, dp);
+        t.bpu(ylp, v0, nv, db, false);
+        t.pull(ylp, v0, nv, sens, nullptr);
     }
+    b->subset_begin[2 * S + 1] = t.L.size();
+    // sweep-kernel plans
     std::map<std::tuple<int, int, int>, std::vector<int>> groups;
-    for (size_t i = 0; i < b->plans.size(); i++) {
-        const et_plan *p = b->plans[i];
+    for (size_t i = 0; i < oldp.size(); i++) {
+        const et_plan *p = oldp[i];
         groups[std::make_tuple(p->R, nthreads_for(p->nx), p->osem_cur_mu ? 1 : 0)].push_back((int)i);
     }
     TableBytes tb;
     std::vector<et_batch::Launch> L;
     for (int s = 0; s < S; s++) {
+        b->subset_begin[2 * s] = L.size();
+        if (oldp.empty()) continue;
         for (int mode : {ET_MODE_PROJECT, ET_MODE_BPU}) {
             if (mode == ET_MODE_BPU) {
                 std::vector<EtRatioArgs> ra;
                 BlockTable bt;
-                for (et_plan *p : b->plans) {
+                for (et_plan *p : oldp) {
                     const int v0 = p->osem_start[s], nv = p->osem_start[s + 1] - v0;
                     const long long VP = (long long)nv * (long long)p->P;
                     ra.push_back({p->osem_meas.p + (size_t)v0 * p->P, p->part.p, p->osem_Dp, VP, p->osem_cur_eps,
@@ -2000,7 +2005,7 @@ void batch_build(et_batch *b, int S, hipStream_t st)
                 std::vector<EtSweepArgs> args;
                 BlockTable bt;
                 for (int i : g.second) {
-                    et_plan *p = b->plans[i];
+                    et_plan *p = oldp[i];
                     const int v0 = p->osem_start[s], nv = p->osem_start[s + 1] - v0;
                     const float *r = p->osem_recs.p + (size_t)v0 * ET_VIEW_REC;
                     const int ts = et_sweep_slab_rows(mode, p->R);
@@ -2022,7 +2027,7 @@ void batch_build(et_batch *b, int S, hipStream_t st)
         }
         std::vector<EtPullArgs> pa;
         BlockTable bt;
-        for (et_plan *p : b->plans) {
+        for (et_plan *p : oldp) {
             const int v0 = p->osem_start[s], nv = p->osem_start[s + 1] - v0;
             const float *r = p->osem_recs.p + (size_t)v0 * ET_VIEW_REC;
             pa.push_back(pull_args(p, r, nv, p->u.p, false, ET_EPI_OSEM, false, false, nullptr, p->x_yf.p,
@@ -2031,37 +2036,43 @@ void batch_build(et_batch *b, int S, hipStream_t st)
         }
         L.push_back({3, 0, 256, 0, tb.put(pa), tb.put(bt.t), (int)pa.size(), bt.total});
     }
+    b->subset_begin[2 * S] = L.size();
     b->drop_graph();
-    b->tables.ensure((align256(tb.bytes.size()) + 3) / 4, false);
-    check_hip(hipMemcpyAsync(b->tables.p, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice, st),
-              niftyrec_error_transfergpu);
-    check_hip(hipStreamSynchronize(st), niftyrec_error_transfergpu);
+    if (!tb.bytes.empty()) {
+        b->tables.ensure((align256(tb.bytes.size()) + 3) / 4, false);
+        check_hip(hipMemcpyAsync(b->tables.p, tb.bytes.data(), tb.bytes.size(), hipMemcpyHostToDevice, st),
+                  niftyrec_error_transfergpu);
+        check_hip(hipStreamSynchronize(st), niftyrec_error_transfergpu);
+    }
+    if (!t.tb.bytes.empty()) yl_upload(b->yl_tables, t, st);
     b->launches = L;
+    b->yl_launches = t.L;
+    b->S = S;
 }
 
 void batch_issue(const et_batch *b, hipStream_t st)
 {
-    if (b->yl) {
-        yl_issue(b->tables.p, b->yl_launches, 0, b->yl_launches.size(), st);
-        return;
-    }
     const char *base = reinterpret_cast<const char *>(b->tables.p);
-    for (const auto &l : b->launches) {
-        const void *args = base + l.args_off, *tab = base + l.tab_off;
-        const et_stream_t s = (et_stream_t)st;
-        switch (l.kind) {
-        case 0:
-        case 2:
-            check_launch(et_launch_sweep_group(l.kind == 0 ? ET_MODE_PROJECT : ET_MODE_BPU, l.R, l.has_mu, l.nthr,
-                                               static_cast<const EtSweepArgs *>(args), tab, l.n, l.blocks, s));
-            break;
-        case 1:
-            check_launch(et_launch_ratio_group(static_cast<const EtRatioArgs *>(args), tab, l.n, l.blocks, s));
-            break;
-        default:
-            check_launch(et_launch_pull_group(ET_EPI_OSEM, static_cast<const EtPullArgs *>(args), tab, l.n, l.blocks,
-                                              s));
+    for (int sub = 0; sub < b->S; sub++) {
+        for (size_t k = b->subset_begin[2 * sub]; k < b->subset_begin[2 * sub + 2]; k++) {
+            const auto &l = b->launches[k];
+            const void *args = base + l.args_off, *tab = base + l.tab_off;
+            const et_stream_t s = (et_stream_t)st;
+            switch (l.kind) {
+            case 0:
+            case 2:
+                check_launch(et_launch_sweep_group(l.kind == 0 ? ET_MODE_PROJECT : ET_MODE_BPU, l.R, l.has_mu, l.nthr,
+                                                   static_cast<const EtSweepArgs *>(args), tab, l.n, l.blocks, s));
+                break;
+            case 1:
+                check_launch(et_launch_ratio_group(static_cast<const EtRatioArgs *>(args), tab, l.n, l.blocks, s));
+                break;
+            default:
+                check_launch(et_launch_pull_group(ET_EPI_OSEM, static_cast<const EtPullArgs *>(args), tab, l.n,
+                                                  l.blocks, s));
+            }
         }
+        yl_issue(b->yl_tables.p, b->yl_launches, b->subset_begin[2 * sub + 1], b->subset_begin[2 * sub + 3], st);
     }
 }
 
@@ -2078,16 +2089,13 @@ void op_batch_osem(et_batch *b, float *const *act, const float *const *sino, con
         any_fp = any_fp || (m && p->mu_check);
     }
     if (any_fp) check_hip(hipStreamSynchronize(st), niftyrec_error_kernel);
-    // the row-lane kernels when every plan can take them, else the sweep kernels for all
-    bool all_yl = true;
-    for (size_t i = 0; i < b->plans.size(); i++) all_yl = all_yl && b->plans[i]->yl && mu && mu[i];
     for (size_t i = 0; i < b->plans.size(); i++) {
         et_plan *p = b->plans[i];
         const float *m = mu ? mu[i] : nullptr;
-        osem_prepare(p, act[i], sino[i], m, S, eps, mu_key_read(p, m), all_yl);
+        osem_prepare(p, act[i], sino[i], m, S, eps, mu_key_read(p, m));
     }
     const std::vector<const void *> key = batch_key(b, S, eps);
-    if (key != b->key || (b->launches.empty() && b->yl_launches.empty())) {
+    if (key != b->key || b->subset_begin.empty()) {
         batch_build(b, S, st);
         b->key = key;
     }
