@@ -245,7 +245,15 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 
     for (int z = zb; z < ze; z++) {
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
-        float h[NR][RPL];
+        // the plane's attenuation factors first: their latency passes under the
+        // gather and the column pass (every wave of the block reaches the
+        // accumulation at the same time, after the barrier)
+        float e[NR][RPL];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = r * W + slot;
+            if (xc < a.nx) yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+        }
         if constexpr (R > 0) {
             float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
 #pragma unroll
@@ -263,7 +271,12 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const int xc = r * W + slot;
-                if (xc < a.nx) yl_row_pass<R, RPL>(plane, CS, xc, row0, kz, h[r]);
+                if (xc < a.nx) {
+                    float h[RPL];
+                    yl_row_pass<R, RPL>(plane, CS, xc, row0, kz, h);
+#pragma unroll
+                    for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
+                }
             }
             if (a.yl_nbuf == 1) __syncthreads();  // one plane: the next writes wait for these reads
         } else {
@@ -272,18 +285,11 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
                 const int xc = r * W + slot;
                 if (xc < a.nx) {
                     const Samp s = fast_sample(fv, (float)xc, z, a.nx, a.nz, dx, dz);
-                    yl_bilinear<RPL>(act + s.off, dx, dz, s, h[r]);
+                    float h[RPL];
+                    yl_bilinear<RPL>(act + s.off, dx, dz, s, h);
+#pragma unroll
+                    for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
                 }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int xc = r * W + slot;
-            if (xc < a.nx) {
-                float e[RPL];
-                yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e);
-#pragma unroll
-                for (int i = 0; i < RPL; i++) kahan(r, i, h[r][i], e[i]);
             }
         }
     }
@@ -336,17 +342,33 @@ __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
         }
     }
     float *uv = a.u + (size_t)v * a.ustride + row0;
+    // attenuation factors one plane ahead (their latency passes under the
+    // previous plane's row pass and stores)
+    auto load_e = [&](int z, float (&e)[NR][RPL]) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = r * W + slot;
+            if (xc < a.nx && z < ze) yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+        }
+    };
+    float en[NR][RPL];
+    load_e(zb, en);
     for (int z = zb; z < ze; z++) {
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
         float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
+        float ec[NR][RPL];
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+            for (int i = 0; i < RPL; i++) ec[r][i] = en[r][i];
+        load_e(z + 1, en);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int xc = r * W + slot;
             if (xc < a.nx) {
-                float e[RPL], w[RPL];
-                yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e);
+                float w[RPL];
 #pragma unroll
-                for (int i = 0; i < RPL; i++) w[i] = sv[r][i] * e[i];
+                for (int i = 0; i < RPL; i++) w[i] = sv[r][i] * ec[r][i];
                 if constexpr (R > 0) {
                     float cp[RPL];
                     yl_column_pass<R, RPL>(w, kz + K, cp);
