@@ -102,6 +102,22 @@ __device__ __forceinline__ float gload(const float *p) { return *(g_cfloat *)p; 
 __device__ __forceinline__ float4 gload4(const float *p) { return *(g_cfloat4 *)p; }
 __device__ __forceinline__ void gstore(float *p, float v) { *(g_float *)p = v; }
 __device__ __forceinline__ void gstore4(float *p, float4 v) { *(g_float4 *)p = v; }
+// Streaming loads (non-temporal: data read once, kept from displacing the
+// caches' reused lines).
+typedef float et_f4v __attribute__((ext_vector_type(4)));
+typedef float et_f2v __attribute__((ext_vector_type(2)));
+typedef ET_GLOBAL_AS const et_f4v g_cf4v;
+typedef ET_GLOBAL_AS const et_f2v g_cf2v;
+__device__ __forceinline__ float4 gload4_stream(const float *p)
+{
+    const et_f4v v = __builtin_nontemporal_load((g_cf4v *)p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 gload2_stream(const float *p)
+{
+    const et_f2v v = __builtin_nontemporal_load((g_cf2v *)p);
+    return make_float2(v.x, v.y);
+}
 // Read-only, block-uniform data (view records, PSF taps, argument records)
 // read through the constant address space: scalar loads.
 typedef ET_CONST_AS const float c_cfloat;

@@ -70,6 +70,30 @@ __device__ __forceinline__ void yl_load(const float *p, float *v)
     }
 }
 
+// yl_load through streaming loads (the attenuation factors: read once per
+// subset, and larger than the caches).
+template <int RPL>
+__device__ __forceinline__ void yl_load_stream(const float *p, float *v)
+{
+    if constexpr (RPL % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RPL / 4; q++) {
+            const float4 t = gload4_stream(p + 4 * q);
+            v[4 * q] = t.x;
+            v[4 * q + 1] = t.y;
+            v[4 * q + 2] = t.z;
+            v[4 * q + 3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL / 2; q++) {
+            const float2 t = gload2_stream(p + 2 * q);
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    }
+}
+
 template <int RPL>
 __device__ __forceinline__ void yl_store(float *p, const float *v)
 {
@@ -195,7 +219,9 @@ __device__ __forceinline__ void yl_row_pass(const float *plane, int CS, int xc, 
     }
 }
 
-// Floats of one LDS plane of a row-lane sweep block (column stride rows + 4).
+// Floats of one LDS plane of a row-lane sweep block (column stride rows + 4;
+// rows = 16 RPL is a compile-time constant of each kernel, so the LDS column
+// offsets are instruction immediates).
 __host__ __device__ constexpr int yl_plane_floats(int nx, int R, int rows) { return (nx + 2 * R) * (rows + 4); }
 
 // Zeroes both LDS planes of the block.
@@ -204,6 +230,31 @@ __device__ __forceinline__ void yl_zero_lds(float *lds, int n)
     for (int i = 4 * (int)threadIdx.x; i < n; i += 4 * (int)blockDim.x)
         *reinterpret_cast<float4 *>(lds + i) = make_float4(0.f, 0.f, 0.f, 0.f);
     __syncthreads();
+}
+
+// Samples of a slot's columns computed once per 16-lane row: lane k of the row
+// computes the sample of plane z0 + k / NR, round k % NR (16 / NR planes per
+// batch), and each round takes its offset and weights from that lane with
+// cross-lane permutes (five LDS-crossbar moves instead of the ~25 VALU
+// operations of a sample, which every lane of the row would otherwise repeat).
+struct YlSamp {
+    int off;
+    float w00, w01, w10, w11;
+};
+
+__device__ __forceinline__ YlSamp yl_samp_of(const Samp &s) { return {s.off, s.w00, s.w01, s.w10, s.w11}; }
+
+// The sample held by lane src of this lane's 16-lane row.
+__device__ __forceinline__ Samp yl_samp_from(const YlSamp &h, int src)
+{
+    const int addr = (((int)threadIdx.x & ~15) + src) * 4;
+    Samp s;
+    s.off = __builtin_amdgcn_ds_bpermute(addr, h.off);
+    s.w00 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(h.w00)));
+    s.w01 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(h.w01)));
+    s.w10 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(h.w10)));
+    s.w11 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(h.w11)));
+    return s;
 }
 
 // ------------------------------------------------------------------------------
@@ -221,7 +272,8 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4, W = blockDim.x >> 4;
     const int d = b.x, v = b.y;
     const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
-    const int rows = a.yu, CS = rows + 4, row0 = RPL * lane;
+    constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
+    const int row0 = RPL * lane;
     const int dx = a.yp, dz = a.yp * a.xp;
     const int PF = yl_plane_floats(a.nx, R, rows);
     const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
@@ -243,8 +295,13 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
         acc[r][i] = t;
     };
 
-    for (int z = zb; z < ze; z++) {
+    constexpr int PB = 16 / NR;  // planes per sample batch
+    YlSamp sb;
+    int kb = 0;                  // plane within the batch
+    for (int z = zb; z < ze; z++, kb = kb + 1 == PB ? 0 : kb + 1) {
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
+        if (kb == 0)  // (planes and columns past the ends give samples no round takes)
+            sb = yl_samp_of(fast_sample(fv, (float)((lane % NR) * W + slot), z + lane / NR, a.nx, a.nz, dx, dz));
         // the plane's attenuation factors first: their latency passes under the
         // gather and the column pass (every wave of the block reaches the
         // accumulation at the same time, after the barrier)
@@ -252,7 +309,7 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int xc = r * W + slot;
-            if (xc < a.nx) yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+            if (xc < a.nx) yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
         }
         if constexpr (R > 0) {
             float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
@@ -260,7 +317,7 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
             for (int r = 0; r < NR; r++) {
                 const int xc = r * W + slot;
                 if (xc < a.nx) {
-                    const Samp s = fast_sample(fv, (float)xc, z, a.nx, a.nz, dx, dz);
+                    const Samp s = yl_samp_from(sb, kb * NR + r);
                     float bl[RPL], cp[RPL];
                     yl_bilinear<RPL>(act + s.off, dx, dz, s, bl);
                     yl_column_pass<R, RPL>(bl, kz + K, cp);
@@ -284,7 +341,7 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
             for (int r = 0; r < NR; r++) {
                 const int xc = r * W + slot;
                 if (xc < a.nx) {
-                    const Samp s = fast_sample(fv, (float)xc, z, a.nx, a.nz, dx, dz);
+                    const Samp s = yl_samp_from(sb, kb * NR + r);
                     float h[RPL];
                     yl_bilinear<RPL>(act + s.off, dx, dz, s, h);
 #pragma unroll
@@ -300,7 +357,7 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
         if (d == a.D - 1 && a.bg_line != 0.f) {
             // e^{-A} along the whole ray = the factor of the last plane
             float e[RPL];
-            yl_load<RPL>(a.att + (((size_t)v * a.nz + a.nz - 1) * a.nx + xc) * rows + row0, e);
+            yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + a.nz - 1) * a.nx + xc) * rows + row0, e);
 #pragma unroll
             for (int i = 0; i < RPL; i++) kahan(r, i, a.bg_line, e[i]);
         }
@@ -325,7 +382,8 @@ __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4, W = blockDim.x >> 4;
     const int d = b.x, v = b.y;
     const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
-    const int rows = a.yu, CS = rows + 4, row0 = RPL * lane;
+    constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
+    const int row0 = RPL * lane;
     const int PF = yl_plane_floats(a.nx, R, rows);
     if constexpr (R > 0) yl_zero_lds(lds, a.yl_nbuf * PF);
 
@@ -348,7 +406,7 @@ __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int xc = r * W + slot;
-            if (xc < a.nx && z < ze) yl_load<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+            if (xc < a.nx && z < ze) yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
         }
     };
     float en[NR][RPL];
@@ -428,7 +486,8 @@ __device__ __forceinline__ void yl_factor_body(const EtSweepArgs &a)
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int xc = blockIdx.x * 16 + slot, v = blockIdx.y;
     if (xc >= a.nx) return;  // the whole 16-lane row
-    const int rows = a.yu, row0 = RPL * lane;
+    constexpr int rows = 16 * RPL;  // == a.yu
+    const int row0 = RPL * lane;
     const int dx = a.yp, dz = a.yp * a.xp;
     const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
     const float *mu = a.mu + a.gmu + row0;
@@ -473,16 +532,24 @@ __device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
     const int c = cb * 16 + slot;
     if (c >= a.nx * a.nz) return;
     const int x = c % a.nx, z = c / a.nx;
-    const int rows = a.yu, row0 = RPL * lane;
+    constexpr int rows = 16 * RPL;  // == a.yu
+    const int row0 = RPL * lane;
     const int dx = rows, dz = rows * a.xp;
     float acc[RPL];
 #pragma unroll
     for (int i = 0; i < RPL; i++) acc[i] = 0.f;
+    YlSamp sb;
     for (int v = 0; v < a.V; v++) {
-        const float *rec = a.views + (size_t)v * ET_VIEW_REC;
-        const float qx = pos2(cload(rec + 12), (float)x, cload(rec + 14), (float)z, cload(rec + 15));
-        const float qz = pos2(cload(rec + 20), (float)x, cload(rec + 22), (float)z, cload(rec + 23));
-        const Samp s = make_samp(qx, qz, a.nx, a.nz, dx, dz);
+        if ((v & 15) == 0) {
+            // lane k of the row: the sample of view v + k (views past the
+            // subset give samples no view takes)
+            const int vk = min(v + lane, a.V - 1);
+            const float *rec = a.views + (size_t)vk * ET_VIEW_REC;
+            const float qx = pos2(gload(rec + 12), (float)x, gload(rec + 14), (float)z, gload(rec + 15));
+            const float qz = pos2(gload(rec + 20), (float)x, gload(rec + 22), (float)z, gload(rec + 23));
+            sb = yl_samp_of(make_samp(qx, qz, a.nx, a.nz, dx, dz));
+        }
+        const Samp s = yl_samp_from(sb, v & 15);
         float t[RPL];
         yl_bilinear<RPL>(a.u + (size_t)v * a.ustride + row0 + s.off, dx, dz, s, t);
 #pragma unroll
