@@ -220,6 +220,19 @@ int et_plan_destroy(et_plan *plan);
 /* act or mu may be NULL (not both). */
 int et_plan_project(et_plan *plan, const float *act, const float *mu, float *sino,
                     int truncate, void *stream);
+/* et_plan_project without allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory during the call (code under HIP
+ * graph capture, tight serving loops).  act and sino are required, mu may be
+ * NULL; the plan must be axial with zero backgrounds.  The temporaries (the
+ * padded copies of act and mu) live in a static device array of 32 MiB per
+ * GPU, held until the process ends; plans whose copies exceed it return
+ * niftyrec_error_parameters (each copy holds (nx+2)(nz+2) padded columns of
+ * about ny + 16 floats: 128^3 and 160x96x160 fit with mu, 256^3 does not).
+ * One depth chunk per view: the same kernels and results as et_plan_project
+ * after et_plan_set_tuning(plan, 1, 0).  Calls share the array: calls that
+ * overlap on one GPU race. */
+int et_plan_project_noalloc(et_plan *plan, const float *act, const float *mu, float *sino,
+                            int truncate, void *stream);
 /* adjoint = 0: the reference backprojector; 1: exact transpose of et_plan_project. */
 int et_plan_backproject(et_plan *plan, const float *sino, const float *mu, float *out,
                         int adjoint, int truncate, void *stream);

@@ -63,6 +63,7 @@ SIGNATURES = {
     "et_plan_create": (c_int, [P(c_void_p), c_int, c_int, c_int, fp, c_int, c_int, fp, c_int, c_float, c_float]),
     "et_plan_destroy": (c_int, [c_void_p]),
     "et_plan_project": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "et_plan_project_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "et_plan_backproject": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "et_plan_gradient_attenuation": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                              c_void_p]),

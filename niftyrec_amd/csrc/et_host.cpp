@@ -2129,6 +2129,66 @@ void op_batch_osem(et_batch *b, float *const *act, const float *const *sino, con
 // Exported C ABI
 // ===========================================================================
 
+// ---------------------------------------------------------------------------
+// Projection without allocation.  Its only temporaries, the YF copies of the
+// activity and of the attenuation map, live in `scratch`, a static device
+// array: every GPU the library runs on holds its own copy until the process
+// ends.  Host code never passes `scratch` itself (in host code the name is a
+// host-side stand-in, not a device address): the first call on a device, with
+// that device current, takes the array's device address from
+// hipGetSymbolAddress and keeps it for that device.  Calls share the array, so
+// calls that overlap on one GPU (different streams, or different host threads)
+// race; each call clears what it needs zeroed (with a fill kernel, which graph
+// capture records as a kernel node) before packing.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t ET_NOALLOC_SCRATCH_BYTES = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[ET_NOALLOC_SCRATCH_BYTES];
+
+constexpr int ET_NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[ET_NOALLOC_MAX_DEVICES] = {};
+
+// Device address of this device's `scratch` (the device must be current).
+float *scratch_of(int device)
+{
+    if (device < 0 || device >= ET_NOALLOC_MAX_DEVICES) throw EtError{niftyrec_error_parameters};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+            throw EtError{niftyrec_error_nogpuinstalled};
+        g_scratch_dev[device] = ptr;
+    }
+    return static_cast<float *>(g_scratch_dev[device]);
+}
+
+size_t round256(size_t bytes) { return (bytes + 255) / 256 * 256; }
+
+void op_project_noalloc(et_plan *p, const float *act, const float *mu, float *sino, int truncate)
+{
+    // axial plans with zero backgrounds (the fast path) only: the general
+    // path's rotated copies would not fit
+    if (!p->fast || !act || !sino) throw EtError{niftyrec_error_parameters};
+    const size_t act_bytes = round256(p->yf_elems * sizeof(float));
+    const size_t mu_elems = p->yf_elems + ET_YF_SLACK;
+    const size_t need = act_bytes + (mu ? round256(mu_elems * sizeof(float)) : 0);
+    if (need > ET_NOALLOC_SCRATCH_BYTES) throw EtError{niftyrec_error_parameters};
+    float *base = scratch_of(p->device);
+    float *act_yf = base, *mu_yf = mu ? base + act_bytes / sizeof(float) : nullptr;
+    const et_stream_t st = (et_stream_t)p->st;
+    // guard rows and borders of the YF layouts read as zeros
+    check_launch(et_launch_fill(act_yf, mu ? act_bytes / sizeof(float) + mu_elems : p->yf_elems, 0.f, st));
+    check_launch(et_launch_pack_yf(act, act_yf, p->nx, p->ny, p->nz, p->yp, p->gy, st));
+    if (mu) check_launch(et_launch_pack_yf(mu, mu_yf, p->nx, p->ny, p->nz, p->yp, p->gmu, st));
+    // one depth chunk: the sweep writes the sinogram itself (no partial sums)
+    const EtSweepArgs a = project_args(p, p->d_recs.p, p->V, act_yf, mu_yf, sino, truncate, 1, nullptr, nullptr);
+    check_launch(et_launch_sweep(ET_MODE_PROJECT, p->R, mu ? 1 : 0, 1, a, st));
+}
+
+}  // namespace
+
 extern "C" {
 
 int status_success(int *status)
@@ -2703,6 +2763,12 @@ int et_plan_project(et_plan *plan, const float *act, const float *mu, float *sin
 {
     if (!act && !mu) return niftyrec_error_parameters;
     ET_PLAN_CALL(plan, stream, op_project(plan, act, mu, sino, truncate))
+}
+
+int et_plan_project_noalloc(et_plan *plan, const float *act, const float *mu, float *sino, int truncate,
+                            void *stream)
+{
+    ET_PLAN_CALL(plan, stream, op_project_noalloc(plan, act, mu, sino, truncate))
 }
 
 int et_plan_backproject(et_plan *plan, const float *sino, const float *mu, float *out, int adjoint, int truncate,
