@@ -139,6 +139,34 @@ int et_array_affine(float *image, int *image_size, float *transformed_image,
 int et_array_convolve(float *image, int *image_size, float *out, int *out_size, float *psf,
                       int *psf_size, int GPU);
 
+/* Fisher information of a grid of points, emission-lib/_et_array_interface.h:50-51
+ * (_et_array_interface.cpp:1135-1559; algorithm emission-lib/_et.cpp:531-746).
+ * grid [N, m, N] holds 0 or the 1-based index of a point; fisher (and
+ * fisher_prior, may be NULL) are [n, n] with n = fisher_size[0]:
+ *   F[i][j] = sum over cameras and the overlap of the two points' PSF boxes on
+ *             the detector of PSF_j * PSF_i * projection
+ * (the projection itself, as in the reference, whose inversion by epsilon is
+ * commented out; epsilon is unused).  The first form projects the activity
+ * (truncated); the second takes the projection [N, m, n_cameras] (bkpr_size
+ * gives the grid size).  GPU != 0 selects the reference CUDA prior rule (1
+ * where |d| <= 3 on every axis, i != j), GPU == 0 its CPU rule (1 where
+ * |d| <= 1, the diagonal included).  Status 1 for the reference's parameter
+ * checks, 2 (ET_ERROR_BADGRID) for a repeated or out-of-range grid index,
+ * niftyrec_error_parameters for a missing PSF (the reference dereferences
+ * it), a PSF with fewer planes than the grid's depth, or (first form) a
+ * non-square PSF. */
+int et_array_fisher_grid(float *activity_ptr, int *activity_size, float *cameras, int *cameras_size,
+                         float *psf, int *psf_size, float *grid_ptr, float *fisher_ptr,
+                         float *fisher_prior_ptr, int *fisher_size, float *attenuation,
+                         int *attenuation_size, float epsilon, float background,
+                         float background_attenuation, int GPU);
+int et_array_fisher_grid_projection(float *sinogram_ptr, int *sinogram_size, int *bkpr_size,
+                                    float *cameras, int *cameras_size, float *psf, int *psf_size,
+                                    float *grid_ptr, float *fisher_ptr, float *fisher_prior_ptr,
+                                    int *fisher_size, float *attenuation, int *attenuation_size,
+                                    float epsilon, float background, float background_attenuation,
+                                    int GPU);
+
 /* PSF helpers kept with the reference's (stub) behaviour,
  * emission-lib/_et_array_interface.h:53-54 (_et_array_interface.cpp:896-920). */
 int et_array_calculate_size_psf(unsigned int *psf_size_x, unsigned int *psf_size_y,

@@ -42,6 +42,10 @@ SIGNATURES = {
     "et_array_rotate": (c_int, [fp, ip, fp, fp, fp, c_float, c_int, c_int]),
     "et_array_affine": (c_int, [fp, ip, fp, ip, fp, ip, c_float, c_int]),
     "et_array_convolve": (c_int, [fp, ip, fp, ip, fp, ip, c_int]),
+    "et_array_fisher_grid": (c_int, [fp, ip, fp, ip, fp, ip, fp, fp, fp, ip, fp, ip, c_float, c_float, c_float,
+                                     c_int]),
+    "et_array_fisher_grid_projection": (c_int, [fp, ip, ip, fp, ip, fp, ip, fp, fp, fp, ip, fp, ip, c_float,
+                                                c_float, c_float, c_int]),
     "et_array_calculate_size_psf": (c_int, [P(c_uint), P(c_uint), c_float, c_float, c_float, c_float, c_float,
                                             c_float]),
     "et_array_make_psf": (c_int, [fp, c_uint, c_uint, c_float, c_float, c_float, c_float, c_float, c_float, c_uint]),

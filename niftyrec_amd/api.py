@@ -238,6 +238,42 @@ def convolve(image, psf):
     return out
 
 
+def fisher_grid(activity, grid, cameras, psf, n=None, attenuation=None, background=0.0,
+                background_attenuation=0.0, prior=True, gpu_prior_rule=True):
+    """Fisher information (n, n) of the points of `grid` (nz, ny, nx): 0, or the
+    1-based index of a point; n defaults to the largest index.  The activity is
+    projected first (et_array_fisher_grid).  Returns (fisher, prior or None);
+    gpu_prior_rule selects the reference CUDA prior rule (|d| <= 3, off the
+    diagonal) over its CPU rule (|d| <= 1, with the diagonal)."""
+    L = _lib.load()
+    act, mu, g, psf = _f32(activity), _f32(attenuation), _f32(grid), _f32(psf)
+    n = int(g.max()) if n is None else int(n)
+    cams, cols, V = _cams(cameras)
+    F = np.zeros((n, n), np.float32)
+    P = np.zeros((n, n), np.float32) if prior else None
+    st = L.et_array_fisher_grid(_p(act), _size(act), _p(cams), (ctypes.c_int * 2)(V, cols), _p(psf), _size(psf),
+                                _p(g), _p(F), _p(P), (ctypes.c_int * 2)(n, n), _p(mu), _size(mu), 0.0,
+                                float(background), float(background_attenuation), int(bool(gpu_prior_rule)))
+    _lib.check(st, "fisher_grid")
+    return F, P
+
+
+def fisher_grid_projection(projection, grid, cameras, psf, n=None, prior=True, gpu_prior_rule=True):
+    """fisher_grid from a given projection (V, ny, nx) of the activity
+    (et_array_fisher_grid_projection); grid (nz, ny, nx)."""
+    L = _lib.load()
+    s, g, psf = _f32(projection), _f32(grid), _f32(psf)
+    n = int(g.max()) if n is None else int(n)
+    cams, cols, V = _cams(cameras)
+    F = np.zeros((n, n), np.float32)
+    P = np.zeros((n, n), np.float32) if prior else None
+    st = L.et_array_fisher_grid_projection(_p(s), _size(s), _size(g), _p(cams), (ctypes.c_int * 2)(V, cols),
+                                           _p(psf), _size(psf), _p(g), _p(F), _p(P), (ctypes.c_int * 2)(n, n), None,
+                                           (ctypes.c_int * 3)(0, 0, 0), 0.0, 0.0, 0.0, int(bool(gpu_prior_rule)))
+    _lib.check(st, "fisher_grid_projection")
+    return F, P
+
+
 class Plan:
     """Device-resident projector for torch tensors (cuda / ROCm device memory).
 

@@ -1677,6 +1677,131 @@ int project_host(float *activity, int *activity_size, float *sinogram, int *sino
     });
 }
 
+// Fisher information of a grid of points, from an activity (its truncated
+// projection first, from_projection false) or from a given projection.
+// Reference emission-lib/_et_array_interface.cpp:1135-1559 (validation and
+// camera expansion) and emission-lib/_et.cpp:531-746 (et_fisher_grid).
+//   vol_size   the activity size (from an activity) or the grid size
+//              (bkpr_size, from a projection), [nx, ny, nz]
+//   input      activity [nx, ny, nz] or projection [snx, sny, V] (input_size)
+//   grid       [nx, ny, nz]: 0, or the 1-based index of a grid point
+//   fisher, prior  [n, n], n = fisher_size[0]; prior may be null
+// The prior rule follows the reference's dispatch on the GPU flag: the CUDA
+// rule (|d| <= 3 off the diagonal, _et.cpp:1633-1650) for GPU != 0, the CPU
+// rule (|d| <= 1 with the diagonal, _et.cpp:703-719) for GPU == 0.
+int fisher_host(bool from_projection, const float *input, const int *input_size, const int *vol_size,
+                const float *cameras, const int *cameras_size, const float *psf, const int *psf_size,
+                const float *grid, float *fisher, float *prior, const int *fisher_size, const float *attenuation,
+                const int *attenuation_size, float background, float background_attenuation, int GPU)
+{
+    // Validation in the reference's order and with its status (1) where the
+    // reference checks (:1146-1199, :1356-1410).
+    const bool no_psf = is_absent(psf_size) || psf == nullptr;
+    const bool no_mu = is_absent(attenuation_size) || attenuation == nullptr;
+    const int V = cameras_size[0], cols = cameras_size[1];
+    if (!(cols == 1 || cols == 3 || cols == 6)) return niftyrec_error_unspecified;
+    const bool dims2 = vol_size[2] == 1;
+    if (dims2) {
+        if (vol_size[0] != vol_size[1]) return niftyrec_error_unspecified;
+        if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1)) return niftyrec_error_unspecified;
+    } else {
+        if (!from_projection && vol_size[0] != vol_size[2]) return niftyrec_error_unspecified;
+        if (!no_psf && (psf_size[0] % 2 != 1 || psf_size[1] % 2 != 1 || psf_size[2] != vol_size[0]))
+            return niftyrec_error_unspecified;
+    }
+    // Beyond the reference's checks (where it would read out of bounds or
+    // dereference a null PSF): a PSF is required and covers every depth of the
+    // grid, the projection holds every camera, the sizes are positive.
+    const int nx = vol_size[0], ny = vol_size[1], nz = vol_size[2], n = fisher_size[0];
+    if (no_psf || V <= 0 || nx <= 0 || ny <= 0 || nz <= 0 || n < 0 || psf_size[2] < nz)
+        return niftyrec_error_parameters;
+    if (from_projection && (input_size[0] <= 0 || input_size[1] <= 0 || input_size[2] != V))
+        return niftyrec_error_parameters;
+    if (!from_projection) {
+        // the projection runs through this library's projector: a 3-D [N, m, N]
+        // volume with a square PSF and a matching attenuation map
+        if (dims2 || psf_size[0] != psf_size[1] || nx > ET_MAX_NX) return niftyrec_error_parameters;
+        if (!no_mu && (attenuation_size[0] != nx || attenuation_size[1] != ny || attenuation_size[2] != nz))
+            return niftyrec_error_parameters;
+    }
+    Cameras cams;
+    // centres: the activity size, or [N, m, N] from the projection's size (:1412-1414)
+    const int ref_size_proj[3] = {from_projection ? input_size[0] : 0, from_projection ? input_size[1] : 0,
+                                  from_projection ? input_size[0] : 0};
+    int rc = expand_cameras(cameras, V, cols, from_projection ? ref_size_proj : vol_size, cams);
+    if (rc) return rc;
+    // grid points (:583-602), x-, then y-, then z-major scan
+    std::vector<int> c0(3 * (size_t)std::max(n, 1), -1);
+    for (int x = 0; x < nx; x++)
+        for (int y = 0; y < ny; y++)
+            for (int z = 0; z < nz; z++) {
+                const float g = grid[(size_t)z * ny * nx + (size_t)y * nx + x];
+                if (g == 0.f) continue;
+                const int k = (int)(g - 1);
+                if (k < 0 || k >= n || c0[3 * (size_t)k] != -1) return 2;  // ET_ERROR_BADGRID
+                c0[3 * (size_t)k] = x;
+                c0[3 * (size_t)k + 1] = y;
+                c0[3 * (size_t)k + 2] = z;
+            }
+    if (n == 0) return niftyrec_success;
+    // grid points in each camera frame: XYZ rotation about the camera's centre,
+    // M (x, y, z, 1) left to right in float, truncated (:617-623)
+    std::vector<int> rc3((size_t)V * n * 3);
+    for (int cam = 0; cam < V; cam++) {
+        float M[16];
+        rotation_matrix(M, cams.ang[cam], cams.ang[V + cam], cams.ang[2 * V + cam], cams.ctr[cam],
+                        cams.ctr[V + cam], cams.ctr[2 * V + cam], 0);
+        for (int k = 0; k < n; k++) {
+            const float p0 = (float)c0[3 * (size_t)k], p1 = (float)c0[3 * (size_t)k + 1],
+                        p2 = (float)c0[3 * (size_t)k + 2];
+            int *o = &rc3[((size_t)cam * n + k) * 3];
+            for (int r = 0; r < 3; r++) o[r] = (int)(M[4 * r] * p0 + M[4 * r + 1] * p1 + M[4 * r + 2] * p2 + M[4 * r + 3]);
+        }
+    }
+    int st = select_device();
+    if (st) return st;
+    return guarded([&]() -> int {
+        HostCall hc;
+        hc.init();
+        const size_t npsf = (size_t)psf_size[0] * psf_size[1] * psf_size[2];
+        EtFisherArgs a;
+        a.n = n;
+        a.V = V;
+        a.px = psf_size[0];
+        a.py = psf_size[1];
+        a.nz = nz;
+        a.prior_rule = GPU ? 3 : 1;
+        a.psf = hc.upload(psf, npsf);
+        a.coords = reinterpret_cast<const int *>(hc.upload(reinterpret_cast<const float *>(rc3.data()), rc3.size()));
+        a.coords0 = reinterpret_cast<const int *>(hc.upload(reinterpret_cast<const float *>(c0.data()), 3 * (size_t)n));
+        if (from_projection) {
+            a.snx = input_size[0];
+            a.sny = input_size[1];
+            a.sino = hc.upload(input, (size_t)a.snx * a.sny * V);
+        } else {
+            // the activity's projection, truncated (_et.cpp:563)
+            et_plan plan;
+            plan.st = hc.st;
+            int r = build_plan(&plan, nx, ny, nz, cams, psf, psf_size[0], background, background_attenuation);
+            if (r) return r;
+            const size_t nv = (size_t)nx * ny * nz;
+            const float *d_act = hc.upload(input, nv);
+            const float *d_mu = no_mu ? nullptr : hc.upload(attenuation, nv);
+            float *d_sino = hc.alloc((size_t)nx * ny * V);
+            op_project(&plan, d_act, d_mu, d_sino, 1);
+            a.snx = nx;
+            a.sny = ny;
+            a.sino = d_sino;
+        }
+        a.fisher = hc.alloc((size_t)n * n);
+        a.prior = prior ? hc.alloc((size_t)n * n) : nullptr;
+        check_launch(et_launch_fisher_pairs(a, (et_stream_t)hc.st));
+        hc.download(fisher, a.fisher, (size_t)n * n);
+        if (prior) hc.download(prior, a.prior, (size_t)n * n);
+        return niftyrec_success;
+    });
+}
+
 int backproject_host(float *sino, int *sino_size, float *bkpr, int *bkpr_size, float *cameras, int *cameras_size,
                      float *psf, int *psf_size, float *attenuation, int *attenuation_size, float background,
                      float background_attenuation, int truncate, bool adjoint)
@@ -2322,6 +2447,37 @@ int et_array_project(float *activity, int *activity_size, float *sinogram, int *
     (void)GPU;
     return project_host(activity, activity_size, sinogram, sinogram_size, cameras, cameras_size, psf, psf_size,
                         attenuation, attenuation_size, background, background_attenuation, truncate_negative_values);
+}
+
+int et_array_fisher_grid(float *activity_ptr, int *activity_size, float *cameras, int *cameras_size, float *psf,
+                         int *psf_size, float *grid_ptr, float *fisher_ptr, float *fisher_prior_ptr, int *fisher_size,
+                         float *attenuation, int *attenuation_size, float epsilon, float background,
+                         float background_attenuation, int GPU)
+{
+    (void)epsilon;  // unused by the reference too (its use is commented out, _et.cpp:576-578)
+    if (!activity_ptr || !activity_size || !cameras || !cameras_size || !psf_size || !grid_ptr || !fisher_ptr ||
+        !fisher_size)
+        return niftyrec_error_parameters;
+    return fisher_host(false, activity_ptr, activity_size, activity_size, cameras, cameras_size, psf, psf_size,
+                       grid_ptr, fisher_ptr, fisher_prior_ptr, fisher_size, attenuation, attenuation_size, background,
+                       background_attenuation, GPU);
+}
+
+int et_array_fisher_grid_projection(float *sinogram_ptr, int *sinogram_size, int *bkpr_size, float *cameras,
+                                    int *cameras_size, float *psf, int *psf_size, float *grid_ptr, float *fisher_ptr,
+                                    float *fisher_prior_ptr, int *fisher_size, float *attenuation,
+                                    int *attenuation_size, float epsilon, float background,
+                                    float background_attenuation, int GPU)
+{
+    (void)epsilon;
+    (void)attenuation;  // the projection is given: attenuation and backgrounds play no part
+    (void)attenuation_size;
+    if (!sinogram_ptr || !sinogram_size || !bkpr_size || !cameras || !cameras_size || !psf_size || !grid_ptr ||
+        !fisher_ptr || !fisher_size)
+        return niftyrec_error_parameters;
+    return fisher_host(true, sinogram_ptr, sinogram_size, bkpr_size, cameras, cameras_size, psf, psf_size, grid_ptr,
+                       fisher_ptr, fisher_prior_ptr, fisher_size, nullptr, nullptr, background,
+                       background_attenuation, GPU);
 }
 
 int et_array_backproject(float *sino, int *sino_size, float *bkpr, int *bkpr_size, float *cameras, int *cameras_size,

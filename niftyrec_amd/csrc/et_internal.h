@@ -188,3 +188,17 @@ int et_launch_gather_planes_t(float *dst, const float *src, const int *order, in
                               et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).
 int et_sweep_slab_rows(int mode, int R);
+
+// Fisher information of a grid of points (reference emission-lib/_et.cpp:531-746).
+struct EtFisherArgs {
+    const int *coords;   // [V][n][3] grid points rotated into each camera frame (truncated)
+    const int *coords0;  // [n][3] grid points in the image frame (prior)
+    const float *sino;   // [V][sny][snx] projection of the activity
+    const float *psf;    // [planes][py][px]
+    float *fisher;       // [n][n]
+    float *prior;        // [n][n] or null
+    int n, V, snx, sny, px, py, nz;
+    int prior_rule;      // 1: |d| <= 1 with the diagonal; 3: |d| <= 3 without it
+};
+// Every pair i <= j of grid points in one thread: F[i][j] = F[j][i] (and the prior).
+int et_launch_fisher_pairs(const EtFisherArgs &a, et_stream_t stream);

@@ -837,6 +837,64 @@ int et_launch_conv_axis(const float *src, float *dst, int nx, int ny, int nz, lo
 
 int et_launch_truncate(float *buf, size_t n, et_stream_t stream) { return et_launch_clamp_min(buf, n, 0.f, stream); }
 
+// ------------------------------------------------------------------------------
+// Fisher information of a grid of points.  Thread (i, j), j >= i, of a 16 x 16
+// tile of the upper triangle (tiles below the diagonal exit at once) sums, for
+// every camera, the products of the two points' PSFs over the overlap of
+// their boxes on the detector, weighted by the projection:
+//   F[i][j] = sum_cam sum_box PSF_j * PSF_i * sino
+// in the reference's order (cameras, then box x, then box y, each product
+// (PSF_j * PSF_i) * sino added to the running sum without contraction), so
+// the result matches the reference CPU loop bit for bit.  Reference
+// emission-lib/_et.cpp:627-698 (offsets of the two PSFs in the box are
+// max(ix, jx) - ix etc., the four cases of :655-679); the prior :703-719
+// (rule 1) and :1633-1650 (rule 3); symmetrisation :722-731.
+// ------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fisher_pairs(EtFisherArgs a)
+{
+#pragma clang fp contract(off)
+    if (blockIdx.x < blockIdx.y) return;  // tile below the diagonal
+    const int i = blockIdx.y * 16 + (threadIdx.x >> 4), j = blockIdx.x * 16 + (threadIdx.x & 15);
+    if (i >= a.n || j >= a.n || j < i) return;
+    const int hx = (a.px - 1) / 2, hy = (a.py - 1) / 2;
+    float f = 0.f;
+    for (int cam = 0; cam < a.V; cam++) {
+        const int *ci = a.coords + ((size_t)cam * a.n + i) * 3, *cj = a.coords + ((size_t)cam * a.n + j) * 3;
+        const int ix = ci[0], iy = ci[1], iz = ci[2], jx = cj[0], jy = cj[1], jz = cj[2];
+        const int bx = a.px - abs(jx - ix), by = a.py - abs(jy - iy);
+        if (!(bx > 0 && by > 0 && iz >= 0 && jz >= 0 && iz < a.nz && jz < a.nz)) continue;
+        const int mx = max(ix, jx), my = max(iy, jy);
+        const float *Pi = a.psf + (size_t)iz * a.px * a.py + (my - iy) * a.px + (mx - ix);
+        const float *Pj = a.psf + (size_t)jz * a.px * a.py + (my - jy) * a.px + (mx - jx);
+        const float *s = a.sino + (size_t)cam * a.snx * a.sny;
+        for (int x = 0; x < bx; x++) {
+            const int qx = mx - hx + x;
+            if (qx < 0 || qx >= a.snx) continue;  // every y of this x lies outside
+            for (int y = 0; y < by; y++) {
+                const int qy = my - hy + y;
+                if (qy >= 0 && qy < a.sny) f = f + (Pj[y * a.px + x] * Pi[y * a.px + x]) * s[(size_t)qy * a.snx + qx];
+            }
+        }
+    }
+    a.fisher[(size_t)i * a.n + j] = f;
+    a.fisher[(size_t)j * a.n + i] = f;
+    if (a.prior) {
+        const int *gi = a.coords0 + (size_t)i * 3, *gj = a.coords0 + (size_t)j * 3;
+        const int lim = a.prior_rule;
+        const bool near = abs(gi[0] - gj[0]) <= lim && abs(gi[1] - gj[1]) <= lim && abs(gi[2] - gj[2]) <= lim;
+        const float p = (near && !(lim == 3 && i == j)) ? 1.f : 0.f;
+        a.prior[(size_t)i * a.n + j] = p;
+        a.prior[(size_t)j * a.n + i] = p;
+    }
+}
+
+int et_launch_fisher_pairs(const EtFisherArgs &a, et_stream_t stream)
+{
+    const int t = (a.n + 15) / 16;
+    hipLaunchKernelGGL(k_fisher_pairs, dim3(t, t), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), a);
+    return (int)hipGetLastError();
+}
+
 int et_launch_fill(float *buf, size_t n, float value, et_stream_t stream)
 {
     hipLaunchKernelGGL(k_fill, dim3(nblk(n, 256)), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), buf, n, value);

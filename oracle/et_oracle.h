@@ -56,7 +56,11 @@
                                 const REAL *cams, const REAL *centers, int V,             \
                                 const REAL *psf, int h, REAL bg, const REAL *bgimg,       \
                                 REAL bga, int truncate, int rotate_partial, REAL *sino,   \
-                                REAL *partial);
+                                REAL *partial);                                           \
+    int orc_fisher_grid_##S(const REAL *sino, int snx, int sny, const REAL *grid, int nx,   \
+                            int ny, int nz, int n, const REAL *cams, const REAL *centers,  \
+                            int V, const REAL *psf, int px, int py, int prior_rule,        \
+                            REAL *fisher, REAL *prior);
 
 ORC_DECLARE(float, f32)
 ORC_DECLARE(double, f64)

@@ -217,3 +217,33 @@ def project_partial(act, mu, cameras, psf=None, bg=0.0, bgimg=None, bga=0.0, tru
         _ptr(act, ct), _ptr(mu, ct), nx, ny, nz, _ptr(cams, ct), _ptr(centers, ct), V, _ptr(psf, ct), h,
         ct(bg), _ptr(bgimg, ct), ct(bga), int(truncate), int(rotate_partial), _ptr(sino, ct), _ptr(part, ct))
     return sino, part
+
+
+def fisher_grid(sino, grid, cameras, psf, n=None, prior_rule=1, centers=None, dtype=np.float32):
+    """Fisher information matrix (n, n) of the grid points of `grid` (nz, ny, nx)
+    (0, or the 1-based point index) from the projection `sino` (V, sny, snx),
+    and the prior matrix (prior_rule 1: CPU rule, 3: GPU rule; 0: None).
+    psf (nz, py, px).  Returns (status, fisher, prior)."""
+    sino, grid, psf = _prep(dtype, sino, grid, psf)
+    nz, ny, nx = grid.shape
+    V, sny, snx = sino.shape
+    n = int(grid.max()) if n is None else int(n)
+    cams, centers, V2 = _geom((nx, sino.shape[1], nx), cameras, centers, dtype)
+    assert V2 == V
+    fisher = np.zeros((n, n), dtype=dtype)
+    prior = np.zeros((n, n), dtype=dtype) if prior_rule else None
+    ct = _ctype(dtype)
+    st = getattr(lib(), "orc_fisher_grid_" + _suffix(dtype))(
+        _ptr(sino, ct), snx, sny, _ptr(grid, ct), nx, ny, nz, n, _ptr(cams, ct), _ptr(centers, ct), V,
+        _ptr(psf, ct), psf.shape[2], psf.shape[1], int(prior_rule), _ptr(fisher, ct), _ptr(prior, ct))
+    return st, fisher, prior
+
+
+def fisher_grid_activity(act, mu, grid, cameras, psf, n=None, bg=0.0, bga=0.0, prior_rule=1, dtype=np.float32):
+    """Fisher grid from an activity: the projection (truncated, as the reference
+    computes it, emission-lib/_et.cpp:563) followed by fisher_grid.  Centres
+    from the activity size."""
+    sino = project(act, mu, cameras, psf, bg, bga, 1, None, dtype)
+    nz, ny, nx = act.shape
+    centers = default_centers(act.shape, sino.shape[0], dtype)
+    return fisher_grid(sino, grid, cameras, psf, n, prior_rule, centers, dtype)

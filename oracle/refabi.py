@@ -137,3 +137,42 @@ def rotate(lib, img, angles, centers, bg=0.0, order=0, gpu=0):
     c = np.ascontiguousarray(centers, np.float32)
     st = lib.et_array_rotate(_p(img), _sz(img).ctypes.data_as(_I), _p(out), _p(a), _p(c), bg, order, gpu)
     return st, out
+
+
+def _fisher_argtypes(lib):
+    f, c = ctypes.c_float, ctypes.c_int
+    lib.et_array_fisher_grid.restype = ctypes.c_int
+    lib.et_array_fisher_grid.argtypes = [_F, _I, _F, _I, _F, _I, _F, _F, _F, _I, _F, _I, f, f, f, c]
+    lib.et_array_fisher_grid_projection.restype = ctypes.c_int
+    lib.et_array_fisher_grid_projection.argtypes = [_F, _I, _I, _F, _I, _F, _I, _F, _F, _F, _I, _F, _I, f, f, f, c]
+
+
+def fisher_grid(lib, act, mu, grid, cameras, psf, n=None, bg=0.0, bga=0.0, gpu=0, prior=True, eps=0.0):
+    """et_array_fisher_grid: returns (status, fisher (n, n), prior (n, n) or None)."""
+    _fisher_argtypes(lib)
+    act, mu, grid, psf = _f(act), _f(mu), _f(grid), _f(psf)
+    n = int(grid.max()) if n is None else int(n)
+    cams, csz = _cams(cameras)
+    fisher = np.zeros((n, n), np.float32)
+    pr = np.zeros((n, n), np.float32) if prior else None
+    st = lib.et_array_fisher_grid(
+        _p(act), _sz(act).ctypes.data_as(_I), _p(cams), csz.ctypes.data_as(_I), _p(psf),
+        _sz(psf).ctypes.data_as(_I), _p(grid), _p(fisher), _p(pr), np.array([n, n], np.int32).ctypes.data_as(_I),
+        _p(mu), _sz(mu).ctypes.data_as(_I), eps, bg, bga, gpu)
+    return st, fisher, pr
+
+
+def fisher_grid_projection(lib, sino, grid, cameras, psf, n=None, mu=None, bg=0.0, bga=0.0, gpu=0, prior=True,
+                           eps=0.0):
+    """et_array_fisher_grid_projection: sino (V, ny, nx), grid (nz, ny, nx)."""
+    _fisher_argtypes(lib)
+    sino, mu, grid, psf = _f(sino), _f(mu), _f(grid), _f(psf)
+    n = int(grid.max()) if n is None else int(n)
+    cams, csz = _cams(cameras)
+    fisher = np.zeros((n, n), np.float32)
+    pr = np.zeros((n, n), np.float32) if prior else None
+    st = lib.et_array_fisher_grid_projection(
+        _p(sino), _sz(sino).ctypes.data_as(_I), _sz(grid).ctypes.data_as(_I), _p(cams), csz.ctypes.data_as(_I),
+        _p(psf), _sz(psf).ctypes.data_as(_I), _p(grid), _p(fisher), _p(pr),
+        np.array([n, n], np.int32).ctypes.data_as(_I), _p(mu), _sz(mu).ctypes.data_as(_I), eps, bg, bga, gpu)
+    return st, fisher, pr

@@ -109,10 +109,67 @@ def main():
         rot["rotated_%d" % order] = out
     np.savez_compressed(os.path.join(HERE, "rotate.npz"), **rot)
     manifest["cases"]["rotate"] = {"background": 0.3, "sha256": {k: sha(v) for k, v in rot.items()}}
+    fisher_cases(R, manifest)
     with open(os.path.join(HERE, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
     print("wrote", len(manifest["cases"]), "cases")
 
 
+def grid_volume(shape, n_points, n, seed):
+    """Grid volume (nz, ny, nx): n_points voxels hold distinct 1-based indices
+    drawn from 1..n (indices left out keep the reference's (-1,-1,-1) point)."""
+    rng = np.random.default_rng(seed)
+    g = np.zeros(shape, np.float32)
+    vox = rng.choice(g.size, n_points, replace=False)
+    g.ravel()[vox] = rng.choice(np.arange(1, n + 1), n_points, replace=False).astype(np.float32)
+    return g
+
+
+def fisher_cases(R, manifest):
+    """et_array_fisher_grid (from an activity: projection, then the pair sums)
+    and et_array_fisher_grid_projection (from a given projection, cameras about
+    three axes, a 5 x 3 PSF), both with the CPU prior; and the status of a grid
+    with a repeated index."""
+    act, mu = volume((16, 16, 16), 300)
+    rng = np.random.default_rng(300)
+    cams = rng.uniform(0, 2 * np.pi, 6).astype(np.float32)
+    psf = gaussian_psf(16, 5, 0.6, 1.5)
+    grid = grid_volume((16, 16, 16), 30, 30, 301)
+    st, F, P = refabi.fisher_grid(R, act, mu, grid, cams, psf, n=30, gpu=0)
+    assert st == 0
+    a = {"activity": act, "attenuation": mu, "cameras": cams, "psf": psf, "grid": grid, "fisher": F, "prior": P}
+    np.savez_compressed(os.path.join(HERE, "fisher_act.npz"), **a)
+    manifest["cases"]["fisher_act"] = {"n": 30, "gpu": 0, "sha256": {k: sha(v) for k, v in a.items()}}
+
+    rng = np.random.default_rng(310)
+    sino = (0.5 + rng.random((5, 20, 20))).astype(np.float32)
+    cams3 = rng.uniform(-0.5, 0.5, (3, 5)).astype(np.float32)
+    t = np.arange(5) - 2.0
+    u = np.arange(3) - 1.0
+    psf53 = np.empty((20, 3, 5), np.float32)  # (planes, py, px): px = 5, py = 3
+    for d in range(20):
+        s = 0.7 + 0.05 * d
+        psf53[d] = np.outer(np.exp(-0.5 * (u / s) ** 2), np.exp(-0.5 * (t / s) ** 2))
+    grid = grid_volume((20, 20, 20), 40, 45, 311)
+    st, F, P = refabi.fisher_grid_projection(R, sino, grid, cams3, psf53, n=45, gpu=0)
+    assert st == 0
+    b = {"projection": sino, "cameras": cams3, "psf": psf53, "grid": grid, "fisher": F, "prior": P}
+    np.savez_compressed(os.path.join(HERE, "fisher_proj.npz"), **b)
+    manifest["cases"]["fisher_proj"] = {"n": 45, "gpu": 0, "sha256": {k: sha(v) for k, v in b.items()}}
+
+    bad = grid_volume((16, 16, 16), 10, 10, 320)
+    bad.ravel()[np.flatnonzero(bad)[:2]] = 3.0  # two voxels with index 3
+    st, _, _ = refabi.fisher_grid(R, act, mu, bad, cams, psf, n=10, gpu=0)
+    manifest["fisher_repeated_index_status"] = int(st)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["fisher"]:  # add the Fisher-grid cases to an existing manifest
+        with open(os.path.join(HERE, "manifest.json")) as f:
+            m = json.load(f)
+        fisher_cases(refabi.load_ref(), m)
+        with open(os.path.join(HERE, "manifest.json"), "w") as f:
+            json.dump(m, f, indent=1, sort_keys=True)
+        print("wrote the Fisher-grid cases")
+    else:
+        main()

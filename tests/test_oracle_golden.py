@@ -10,7 +10,7 @@ from oracle import orc
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 MANIFEST = json.load(open(os.path.join(GOLDEN, "manifest.json")))
-CASES = sorted(k for k in MANIFEST["cases"] if k != "rotate")
+CASES = sorted(k for k in MANIFEST["cases"] if k != "rotate" and not k.startswith("fisher"))
 
 
 def _load(name):
@@ -57,3 +57,33 @@ def test_manifest_hashes():
         g = _load(name)
         for k, h in meta["sha256"].items():
             assert hashlib.sha256(np.ascontiguousarray(g[k]).tobytes()).hexdigest() == h, (name, k)
+
+
+def test_golden_fisher_grid_from_activity():
+    """et_array_fisher_grid of the reference CPU build: projection, pair sums
+    and the CPU prior rule, bit for bit."""
+    g = _load("fisher_act")
+    st, F, P = orc.fisher_grid_activity(g["activity"], g["attenuation"], g["grid"], g["cameras"], g["psf"], n=30,
+                                        prior_rule=1)
+    assert st == 0
+    assert np.array_equal(F, g["fisher"])
+    assert np.array_equal(P, g["prior"])
+
+
+def test_golden_fisher_grid_from_projection():
+    """et_array_fisher_grid_projection: cameras about three axes, a 5 x 3 PSF,
+    grid indices left out (their points sit at (-1,-1,-1)), bit for bit."""
+    g = _load("fisher_proj")
+    st, F, P = orc.fisher_grid(g["projection"], g["grid"], g["cameras"], g["psf"], n=45, prior_rule=1)
+    assert st == 0
+    assert np.array_equal(F, g["fisher"])
+    assert np.array_equal(P, g["prior"])
+    assert np.count_nonzero(F) > 100
+
+
+def test_fisher_repeated_index_status():
+    g = _load("fisher_act")
+    bad = np.zeros_like(g["grid"])
+    bad.ravel()[[5, 77, 300]] = [1.0, 3.0, 3.0]
+    st, _, _ = orc.fisher_grid_activity(g["activity"], g["attenuation"], bad, g["cameras"], g["psf"], n=3)
+    assert st == MANIFEST["fisher_repeated_index_status"] == 2

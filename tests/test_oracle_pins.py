@@ -310,3 +310,65 @@ def test_partial_projection_background_image_and_no_activity():
     for z in range(7):
         want = 2.0 * img * np.exp(-mu[z + 1:].sum(axis=0))
         assert np.allclose(part[0, z], want, rtol=1e-13)
+
+
+def _fisher_brute(sino, grid, cams3, psf, n, centers):
+    """F = sum over cameras of A^T diag(s) A, A[q, k] = the PSF of point k at
+    detector pixel q (its plane chosen by the point's rotated depth), rendered
+    point by point in float64: a formulation with no bounding boxes and no
+    pair offsets."""
+    V, sny, snx = sino.shape
+    nz, ny, nx = grid.shape
+    npl, py, px = psf.shape
+    hx, hy = (px - 1) // 2, (py - 1) // 2
+    coords = -np.ones((n, 3), np.int64)
+    for z, y, x in zip(*np.nonzero(grid)):
+        coords[int(grid[z, y, x]) - 1] = (x, y, z)
+    F = np.zeros((n, n))
+    for c in range(V):
+        M = orc.rotation_matrix(cams3[:, c], centers[:, c], 0, dtype=np.float64)
+        A = np.zeros((sny * snx, n))
+        for k in range(n):
+            p = coords[k].astype(np.float64)
+            r = [int(M[i, 0] * p[0] + M[i, 1] * p[1] + M[i, 2] * p[2] + M[i, 3]) for i in range(3)]
+            if not (0 <= r[2] < nz):
+                continue
+            img = np.zeros((sny, snx))
+            for dy in range(-hy, hy + 1):
+                for dx in range(-hx, hx + 1):
+                    qx, qy = r[0] + dx, r[1] + dy
+                    if 0 <= qx < snx and 0 <= qy < sny:
+                        img[qy, qx] = psf[r[2], dy + hy, dx + hx]
+            A[:, k] = img.ravel()
+        F += A.T @ (sino[c].astype(np.float64).ravel()[:, None] * A)
+    return F, coords
+
+
+def test_fisher_grid_brute_force_and_prior_rules():
+    """The Fisher matrix equals the rendered-footprint formulation (cameras
+    about three axes, a non-square PSF, a missing index); the prior rules
+    follow their definitions on the grid coordinates: CPU rule |d| <= 1 on
+    every axis with the diagonal, GPU rule |d| <= 3 off the diagonal."""
+    rng = np.random.default_rng(5)
+    shape = (10, 10, 10)
+    n = 14
+    grid = np.zeros(shape)
+    vox = rng.choice(grid.size, 12, replace=False)
+    grid.ravel()[vox] = rng.choice(np.arange(1, n + 1), 12, replace=False)
+    sino = 0.5 + rng.random((4, 10, 10))
+    cams3 = rng.uniform(-0.6, 0.6, (3, 4))
+    psf = rng.random((10, 3, 5)) + 0.1  # planes, py = 3, px = 5; asymmetric
+    centers = orc.default_centers(shape, 4, np.float64)
+    st, F, P = orc.fisher_grid(sino, grid, cams3, psf, n=n, prior_rule=1, centers=centers, dtype=np.float64)
+    assert st == 0
+    Fb, coords = _fisher_brute(sino, grid, cams3, psf, n, centers)
+    assert np.count_nonzero(Fb) > 20
+    assert np.allclose(F, Fb, rtol=1e-12, atol=1e-12)
+    d = np.abs(coords[:, None, :] - coords[None, :, :]).max(axis=2)
+    assert np.array_equal(P, (d <= 1).astype(float))
+    st, _, P3 = orc.fisher_grid(sino, grid, cams3, psf, n=n, prior_rule=3, centers=centers, dtype=np.float64)
+    assert np.array_equal(P3, ((d <= 3) & ~np.eye(n, dtype=bool)).astype(float))
+    # the check tells a transposed PSF (x and y taps swapped) apart
+    Ft, _ = _fisher_brute(sino, grid, cams3, np.ascontiguousarray(psf[:, :3, :3].transpose(0, 2, 1)), n, centers)
+    F3, _ = _fisher_brute(sino, grid, cams3, psf[:, :3, :3], n, centers)
+    assert not np.allclose(Ft, F3)
