@@ -2,7 +2,7 @@
 PSF 11x11 and attenuation, 8 ordered subsets of 15 views, on one MI355X with
 every input resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batch]
                     [--workload spect128|batch48|joint256|hbm_copy] [--depth-chunks D]
 
 A step is one OSEM iteration: 8 subset updates, each = projection of the
@@ -30,6 +30,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch48 speed-ups of the default line")
     ap.add_argument("--workload", default="spect128")
     ap.add_argument("--depth-chunks", type=int, default=0,
                     help="fix the sweeps' depth chunking (0: timed by the plan); profiler runs fix it "
@@ -37,21 +38,25 @@ def parse():
     return ap.parse_args()
 
 
-# Kernel of each profiled category, as named in the rocprofv3 summaries.
-# Kernel of each profiled category in the OSEM subset loop at this workload (slabs of
-# 16 rows, PSF radius 5, attenuation factors read from the cache), as named in the
-# rocprofv3 summaries.
-KERNEL_OF = {"project": "void k_project<16, 5, true, true, 2>", "bp_rotated_frame": "void k_bpu<16, 5, true, false, true>",
-             "bp_pull": "void k_pull<false, 2>"}
+# Kernels of the profiled categories in the OSEM subset loop, as named in the
+# rocprofv3 summaries: the row-lane sweeps (PSF radius R, RPL rows per lane of
+# a 16-lane row holding a column's whole axial extent, NR adjacent columns per
+# row) and the pull with the OSEM update.
+def kernel_of(cat, R, ny, nx):
+    rpl, nr = -(-ny // 32) * 2, -(-nx // 64)
+    return {"project": "void (anonymous namespace)::k_yl_project<%d, %d, %d>" % (R, rpl, nr),
+            "bp_rotated_frame": "void (anonymous namespace)::k_yl_bpu<%d, %d, %d>" % (R, rpl, nr),
+            "bp_pull": "void (anonymous namespace)::k_yl_pull<%d, 2>" % rpl}.get(cat)
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "osem_spect128_hbm_traffic.txt")
 
 
-def measured_traffic(cat, grid=None):
-    """HBM bytes per launch of the category's kernel from the committed
+def measured_traffic(prefix, grid=None):
+    """HBM bytes per launch of the kernel named `prefix` from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary: the row of that launch grid
     (work-items), else the row with the most dispatches; None when the summary
     lacks the kernel."""
-    prefix = KERNEL_OF.get(cat)
     if prefix is None or not os.path.exists(TRAFFIC_FILE):
         return None
     best = None
@@ -67,46 +72,69 @@ def measured_traffic(cat, grid=None):
     return None if best is None else best[1] * 1e6
 
 
-def sweep_grid(cat, nx, ny, views, depth_chunks):
-    """Work-items of one subset launch of a sweep kernel: axial slabs of 16
-    rows x views x depth chunks blocks of nx rounded up to 64 threads."""
-    if cat not in ("project", "bp_rotated_frame"):
-        return None
-    return ((ny + 15) // 16) * views * depth_chunks * ((nx + 63) // 64 * 64)
+def sweep_grid(cat, nx, nz, views, depth_chunks):
+    """Work-items of one subset launch: a sweep runs views x depth chunks blocks
+    of 16 W threads (W slots of at most 3 adjacent columns, a multiple of 4);
+    the pull ceil(nx nz / 16) blocks of 256."""
+    if cat == "bp_pull":
+        return -(-nx * nz // 16) * 256
+    nr = -(-nx // 64)
+    w = (-(-nx // nr) + 3) // 4 * 4
+    return views * depth_chunks * 16 * w
 
 
-def cpu_baseline(w, subsets):
+def cpu_baseline(w, subsets, y_sub, x_gpu):
     """Reference CPU path (oracle/_ref, single thread as built by the reference)
-    on one subset update of the same workload; falls back to the oracle port."""
+    on one subset update of the same workload (the first ordered subset, from
+    x = 1, measured counts y_sub); falls back to the oracle port.  The same
+    update is checked here: x_gpu (the library's update) against the
+    reference's result and against the double-precision oracle.  Returns
+    (cpu_baseline record, error record)."""
     from oracle import refabi, orc
     a, m, p, c = w["activity"], w["attenuation"], w["psf"], w["cameras"]
     views = c[0::subsets]  # the first ordered subset: 15 views
-    y = np.ones((len(views),) + a.shape[1:], np.float32)
     if os.path.exists(refabi.REF_PATH):
         R = refabi.load_ref()
+        x32 = np.ones_like(a)
         t = time.perf_counter()
-        st, _ = refabi.mlem_step(R, np.ones_like(a), y, m, views, p)
+        st, x32 = refabi.mlem_step(R, x32, y_sub, m, views, p)
         dt = time.perf_counter() - t
         kind, cores = "reference", 1
         assert st == 0
     else:
         t = time.perf_counter()
-        orc.mlem_step(np.ones_like(a), y, m, views, p)
+        x32 = orc.mlem_step(np.ones_like(a), y_sub, m, views, p)
         dt = time.perf_counter() - t
         kind, cores = "port", os.cpu_count()
     it_time = dt * subsets  # one iteration = `subsets` subset updates of this size
-    return {"value": 1.0 / it_time, "unit": "OSEM it/s", "cores": cores, "kind": kind,
+    base = {"value": 1.0 / it_time, "unit": "OSEM it/s", "cores": cores, "kind": kind,
             "sample": "one subset update (et_array_mlem_step on %d of %d views, 128^3, PSF 11, attenuation; "
                       "the reference recomputes the sensitivity) took %.2f s; iteration = %d such updates"
                       % (len(views), len(c), dt, subsets)}
+    x64 = orc.mlem_step(np.ones(a.shape), y_sub, m, views, p, dtype=np.float64)
+
+    def rel(u, v):
+        u, v = u.astype(np.float64), v.astype(np.float64)
+        return float(np.linalg.norm(u - v) / np.linalg.norm(v))
+
+    def mrel(u, v):
+        u, v = u.astype(np.float64), v.astype(np.float64)
+        return float(np.abs(u - v).max() / np.abs(v).max())
+
+    err = {"update": "first ordered subset (%d views) from x = 1, measured counts" % len(views),
+           "relL2_vs_reference": rel(x_gpu, x32), "max_abs_over_peak_vs_reference": mrel(x_gpu, x32),
+           "relL2_vs_fp64": rel(x_gpu, x64), "reference_relL2_vs_fp64": rel(x32, x64),
+           "reference": kind}
+    return base, err
 
 
-def main_batch(args):
-    """48 uneven problems (phantoms.batch_problem), 8 ordered subsets; a step is
-    one OSEM iteration of every problem through et_batch_osem (grouped launches,
-    one hipGraph replay).  Compared with the per-call ABI path (et_array_mlem_step
-    on host buffers, one call per subset) and with the plans run one after
-    another (et_plan_osem)."""
+def run_batch48(steps, warmup, per_call=True):
+    """48 uneven problems (phantoms.batch_problem), 8 ordered subsets: one OSEM
+    iteration of every problem through et_batch_osem (grouped launches, one
+    hipGraph replay), timed over `steps`; the same plans run one after another
+    (et_plan_osem); and the per-call ABI path (et_array_mlem_step on host
+    buffers, one call per subset; sample: subset 0 of every problem, times 8).
+    Returns a dict of the timings and speed-ups."""
     import ctypes
     import torch
     from niftyrec_amd import api, phantoms, _lib
@@ -125,16 +153,16 @@ def main_batch(args):
         M.append(Mu)
         X.append(torch.ones_like(A))
     batch = api.Batch(plans)
-    batch.osem(X, Y, M, subsets=S, iterations=max(args.warmup, 1))
+    batch.osem(X, Y, M, subsets=S, iterations=max(warmup, 1))
     torch.cuda.synchronize()
     for x in X:
         x.fill_(1.0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         batch.osem(X, Y, M, subsets=S, iterations=1)
     torch.cuda.synchronize()
-    t_batch = (time.perf_counter() - t0) / args.steps
+    t_batch = (time.perf_counter() - t0) / steps
     assert all(torch.isfinite(x).all() for x in X)
 
     # plans one after another (same kernels, one problem per launch, no graph)
@@ -146,47 +174,52 @@ def main_batch(args):
         p.osem(x, y, m, subsets=S, iterations=1)
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
+    out = {"problems": N, "subsets": S, "ms_per_iteration": t_batch * 1e3,
+           "plans_sequential_ms_per_iteration": t_seq * 1e3, "speedup_vs_sequential_plans": t_seq / t_batch,
+           "gvoxel_views_per_s": sum(w["activity"].size * len(w["cameras"]) for w in probs) / t_batch / 1e9}
+    if per_call:
+        L = _lib.load()
+        fp = ctypes.POINTER(ctypes.c_float)
+        t_call = 0.0
+        for w, y in zip(probs, Y):
+            a = np.ones_like(w["activity"])
+            ysub = np.ascontiguousarray(y.cpu().numpy()[0::S])
+            cams = np.ascontiguousarray(np.stack([np.zeros_like(w["cameras"][0::S]), w["cameras"][0::S],
+                                                  np.zeros_like(w["cameras"][0::S])]))
+            psf = np.ascontiguousarray(w["psf"])
+            nz, ny, nx = a.shape
+            mu = np.ascontiguousarray(w["attenuation"])
+            K = psf.shape[2]
+            t1 = time.perf_counter()
+            st = L.et_array_mlem_step(a.ctypes.data_as(fp), nx, ny, ysub.ctypes.data_as(fp), ysub.shape[0],
+                                      cams.ctypes.data_as(fp), 1, mu.ctypes.data_as(fp), 1, psf.ctypes.data_as(fp),
+                                      K, K, 0.0, 0.0, 1e-6, 1)
+            t_call += time.perf_counter() - t1
+            assert st == 0, st
+        out.update(per_call_abi_ms_per_iteration=t_call * S * 1e3, speedup_vs_per_call=t_call * S / t_batch,
+                   per_call_sample="subset 0 of all 48 problems through et_array_mlem_step (host buffers), times 8")
+    for p in plans:
+        p.close()
+    return out
 
-    # per-call ABI path: every subset update is one et_array_mlem_step call on
-    # host buffers (upload, projection, two backprojections, download); sample:
-    # subset 0 of every problem, times S for an iteration
-    L = _lib.load()
-    fp = ctypes.POINTER(ctypes.c_float)
-    t_call = 0.0
-    for w, y in zip(probs, Y):
-        a = np.ones_like(w["activity"])
-        ysub = np.ascontiguousarray(y.cpu().numpy()[0::S])
-        cams = np.ascontiguousarray(np.stack([np.zeros_like(w["cameras"][0::S]), w["cameras"][0::S],
-                                              np.zeros_like(w["cameras"][0::S])]))
-        psf = np.ascontiguousarray(w["psf"])
-        nz, ny, nx = a.shape
-        t1 = time.perf_counter()
-        mu = np.ascontiguousarray(w["attenuation"])
-        K = psf.shape[2]
-        st = L.et_array_mlem_step(a.ctypes.data_as(fp), nx, ny, ysub.ctypes.data_as(fp), ysub.shape[0],
-                                  cams.ctypes.data_as(fp), 1, mu.ctypes.data_as(fp), 1, psf.ctypes.data_as(fp),
-                                  K, K, 0.0, 0.0, 1e-6, 1)
-        t_call += time.perf_counter() - t1
-        assert st == 0, st
-    t_call_iter = t_call * S
-    vox_views = sum(w["activity"].size * len(w["cameras"]) for w in probs)
+
+def main_batch(args):
+    """The batch48 workload on its own (see run_batch48): a step is one OSEM
+    iteration of all 48 problems."""
+    from niftyrec_amd import phantoms
+    b = run_batch48(args.steps, args.warmup)
     line = {
         "metric": "batched OSEM iterations/s, 48 uneven SPECT problems",
-        "value": 1.0 / t_batch,
+        "value": 1e3 / b["ms_per_iteration"],
         "unit": "OSEM it/s (all 48 problems)",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_batch * 1e3,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": b["ms_per_iteration"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded ellipsoid phantoms, Poisson counts)",
         "config": {"workload": "batch48: 48 problems, volumes from %s, 60-180 views, PSF 7/11/15, attenuation, "
                                "8 ordered subsets x 1 iteration per step" % phantoms.BATCH_SHAPES,
-                   "problems": N, "subsets": S},
-        "gvoxel_views_per_s": vox_views / t_batch / 1e9,
-        "plans_sequential_ms_per_iteration": t_seq * 1e3,
-        "per_call_abi_ms_per_iteration": t_call_iter * 1e3,
-        "speedup_vs_per_call": t_call_iter / t_batch,
-        "speedup_vs_sequential_plans": t_seq / t_batch,
-        "per_call_sample": "subset 0 of all 48 problems through et_array_mlem_step (host buffers), times 8",
+                   "problems": b["problems"], "subsets": b["subsets"]},
     }
+    line.update({k: v for k, v in b.items() if k not in ("problems", "subsets", "ms_per_iteration")})
     print(json.dumps(line), flush=True)
 
 
@@ -370,7 +403,8 @@ def main():
         "hbm_frac_of_peak": bytes_iter / (ms * 1e-3) / HBM_PEAK,
         "roofline": {"bound": "hbm", "kernel": cat, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(cat, sweep_grid(cat, nx, ny, V // S, dchunks[cat == "bp_rotated_frame"])),
+                     "traffic": measured_traffic(kernel_of(cat, (psf.shape[-1] - 1) // 2, ny, nx),
+                                                 sweep_grid(cat, nx, nz, V // S, dchunks[cat == "bp_rotated_frame"])),
                      "avg_launch_ms": tot_ms / launches, "algorithmic_bytes_per_launch": tot_bytes / launches,
                      "depth_chunks": dchunks[cat == "bp_rotated_frame"]},
         "kernel_ms_per_step": {k: v[1] / n_split for k, v in split.items() if v[0]},
@@ -379,7 +413,20 @@ def main():
                              % (S + 1, launches),
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(w, S)
+        # the first ordered subset's update from x = 1 on the GPU (a plan over
+        # its views), checked in the CPU-baseline leg
+        sub = api.Plan(act.shape, cams[0::S], psf)
+        Xs = torch.ones_like(A)
+        Ys = Y[0::S].contiguous()
+        sub.osem(Xs, Ys, M, subsets=1, iterations=1)
+        torch.cuda.synchronize()
+        line["cpu_baseline"], line["subset_update_error"] = cpu_baseline(w, S, Ys.cpu().numpy(), Xs.cpu().numpy())
+        sub.close()
+    if not args.no_batch:
+        # batches of uneven problems (outside the timed region): speed-ups of one
+        # hipGraph replay of grouped launches over the plans run one after
+        # another and over the per-call ABI
+        line["batch48"] = run_batch48(3, 1)
     print(json.dumps(line), flush=True)
 
 

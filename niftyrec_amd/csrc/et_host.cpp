@@ -2026,9 +2026,14 @@ struct et_batch {
         int n, blocks;
     };
     std::vector<Launch> launches;     // sweep-kernel plans' launches of one OSEM iteration, in order
-    std::vector<YlLaunch> yl_launches;  // row-lane plans' launches (table yl_tables)
+    std::vector<YlLaunch> yl_launches;  // row-lane plans' launches (table yl_tables), group after group
     DevBuf yl_tables;
-    std::vector<size_t> subset_begin;   // subset s: launches [2s], yl_launches [2s+1] (index 2S, 2S+1: ends)
+    std::vector<size_t> yl_group_begin;  // group g: yl_launches [yl_group_begin[g], yl_group_begin[g + 1])
+    std::vector<size_t> subset_begin;   // subset s: launches [subset_begin[s], subset_begin[s + 1])
+    // Side streams and events of the fork / join that runs the row-lane groups'
+    // chains next to each other (each group's launches only depend on its own).
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> ev;         // ev[0]: fork; ev[1 + k]: join of side stream k
     int S = 0;
     std::vector<const void *> key;    // buffers and chunkings the launch list refers to
     hipGraph_t graph = nullptr;
@@ -2046,6 +2051,8 @@ struct et_batch {
     ~et_batch()
     {
         drop_graph();
+        for (hipStream_t x : side) (void)hipStreamDestroy(x);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (own) (void)hipStreamDestroy(own);
     }
 };
@@ -2069,39 +2076,47 @@ std::vector<const void *> batch_key(const et_batch *b, int S, float eps)
 }
 
 // Launch lists of one OSEM iteration.  Plans whose cached data are the
-// row-lane kernels' take those (yl_launches, one table), the others the sweep
-// kernels (launches): per subset, grouped projections, the ratios, grouped
-// rotated-frame backprojections, then the pulls with the multiplicative
-// update, with the same argument records as a plan run alone, so a batch
-// result is bit-identical to its plans' own et_plan_osem.  The two lists are
-// interleaved subset by subset (subset_begin).
+// row-lane kernels' take those (yl_launches, one table; one chain of all
+// subsets per sweep template, yl_group_begin), the others the sweep kernels
+// (launches, subset by subset, subset_begin): per subset, grouped
+// projections, the ratios, grouped rotated-frame backprojections, then the
+// pulls with the multiplicative update, with the same argument records as a
+// plan run alone, so a batch result is bit-identical to its plans' own
+// et_plan_osem.
 void batch_build(et_batch *b, int S, hipStream_t st)
 {
     std::vector<et_plan *> ylp, oldp;
     for (et_plan *p : b->plans) (p->osem_yl ? ylp : oldp).push_back(p);
-    b->subset_begin.assign((size_t)2 * (S + 1), 0);
-    // row-lane plans
+    b->subset_begin.assign((size_t)S + 1, 0);
+    // row-lane plans: one group per sweep template (their sweeps share
+    // launches); a group's chain holds all its subsets in order and depends on
+    // no other group's
+    std::map<std::tuple<int, int, int, int>, std::vector<et_plan *>> ygroups;
+    for (et_plan *p : ylp) ygroups[YlTable::sweep_key(p)].push_back(p);
     YlTable t;
-    const size_t n = ylp.size();
-    for (int s = 0; s < S; s++) {
-        b->subset_begin[2 * s + 1] = t.L.size();
-        if (n == 0) continue;
-        std::vector<int> v0(n), nv(n), dp(n), db(n);
-        std::vector<const float *> sens(n);
-        for (size_t i = 0; i < n; i++) {
-            const et_plan *p = ylp[i];
-            v0[i] = p->osem_start[s];
-            nv[i] = p->osem_start[s + 1] - v0[i];
-            dp[i] = p->osem_Dp;
-            db[i] = p->osem_Db;
-            sens[i] = p->sens.p + (size_t)s * p->yf_elems;
+    b->yl_group_begin.clear();
+    for (auto &g : ygroups) {
+        b->yl_group_begin.push_back(t.L.size());
+        const std::vector<et_plan *> &gp = g.second;
+        const size_t n = gp.size();
+        for (int s = 0; s < S; s++) {
+            std::vector<int> v0(n), nv(n), dp(n), db(n);
+            std::vector<const float *> sens(n);
+            for (size_t i = 0; i < n; i++) {
+                const et_plan *p = gp[i];
+                v0[i] = p->osem_start[s];
+                nv[i] = p->osem_start[s + 1] - v0[i];
+                dp[i] = p->osem_Dp;
+                db[i] = p->osem_Db;
+                sens[i] = p->sens.p + (size_t)s * p->yf_elems;
+            }
+            t.project(gp, v0, nv, dp);
+            t.ratio(gp, v0, nv, dp);
+            t.bpu(gp, v0, nv, db, false);
+            t.pull(gp, v0, nv, sens, nullptr);
         }
-        t.project(ylp, v0, nv, dp);
-        t.ratio(ylp, v0, nv, dp);
-        t.bpu(ylp, v0, nv, db, false);
-        t.pull(ylp, v0, nv, sens, nullptr);
     }
-    b->subset_begin[2 * S + 1] = t.L.size();
+    b->yl_group_begin.push_back(t.L.size());
     // sweep-kernel plans
     std::map<std::tuple<int, int, int>, std::vector<int>> groups;
     for (size_t i = 0; i < oldp.size(); i++) {
@@ -2111,7 +2126,7 @@ void batch_build(et_batch *b, int S, hipStream_t st)
     TableBytes tb;
     std::vector<et_batch::Launch> L;
     for (int s = 0; s < S; s++) {
-        b->subset_begin[2 * s] = L.size();
+        b->subset_begin[s] = L.size();
         if (oldp.empty()) continue;
         for (int mode : {ET_MODE_PROJECT, ET_MODE_BPU}) {
             if (mode == ET_MODE_BPU) {
@@ -2161,7 +2176,7 @@ void batch_build(et_batch *b, int S, hipStream_t st)
         }
         L.push_back({3, 0, 256, 0, tb.put(pa), tb.put(bt.t), (int)pa.size(), bt.total});
     }
-    b->subset_begin[2 * S] = L.size();
+    b->subset_begin[S] = L.size();
     b->drop_graph();
     if (!tb.bytes.empty()) {
         b->tables.ensure((align256(tb.bytes.size()) + 3) / 4, false);
@@ -2175,11 +2190,50 @@ void batch_build(et_batch *b, int S, hipStream_t st)
     b->S = S;
 }
 
+// Side streams (at most ET_BATCH_SIDE, default priority) and the events of
+// the fork / join, made once per batch.
+constexpr int ET_BATCH_SIDE = 4;
+
+void batch_streams(et_batch *b, int groups)
+{
+    const int want = std::min(groups, ET_BATCH_SIDE);
+    while ((int)b->side.size() < want) {
+        hipStream_t x = nullptr;
+        check_hip(hipStreamCreateWithFlags(&x, hipStreamNonBlocking), niftyrec_error_kernel);
+        b->side.push_back(x);
+    }
+    while (b->ev.size() < b->side.size() + 1) {
+        hipEvent_t e = nullptr;
+        check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), niftyrec_error_kernel);
+        b->ev.push_back(e);
+    }
+}
+
+// One OSEM iteration of every plan.  The sweep-kernel plans' launches go on
+// `st`; with more than one row-lane group, group g's chain goes on side stream
+// g % side.size() after a fork from `st`, and `st` joins every side stream at
+// the end (under capture: parallel branches of the graph), so the groups'
+// launches overlap instead of each waiting out the tail of the one before.
 void batch_issue(const et_batch *b, hipStream_t st)
 {
+    const int G = (int)b->yl_group_begin.size() - 1;
+    const bool fork = G > 1 && !b->side.empty();
+    if (fork) {
+        check_hip(hipEventRecord(b->ev[0], st), niftyrec_error_kernel);
+        for (hipStream_t x : b->side) check_hip(hipStreamWaitEvent(x, b->ev[0], 0), niftyrec_error_kernel);
+    }
+    for (int g = 0; g < G; g++)
+        yl_issue(b->yl_tables.p, b->yl_launches, b->yl_group_begin[g], b->yl_group_begin[g + 1],
+                 fork ? b->side[g % b->side.size()] : st);
+    if (fork) {
+        for (size_t k = 0; k < b->side.size(); k++) {
+            check_hip(hipEventRecord(b->ev[1 + k], b->side[k]), niftyrec_error_kernel);
+            check_hip(hipStreamWaitEvent(st, b->ev[1 + k], 0), niftyrec_error_kernel);
+        }
+    }
     const char *base = reinterpret_cast<const char *>(b->tables.p);
     for (int sub = 0; sub < b->S; sub++) {
-        for (size_t k = b->subset_begin[2 * sub]; k < b->subset_begin[2 * sub + 2]; k++) {
+        for (size_t k = b->subset_begin[sub]; k < b->subset_begin[sub + 1]; k++) {
             const auto &l = b->launches[k];
             const void *args = base + l.args_off, *tab = base + l.tab_off;
             const et_stream_t s = (et_stream_t)st;
@@ -2197,7 +2251,6 @@ void batch_issue(const et_batch *b, hipStream_t st)
                                                   l.blocks, s));
             }
         }
-        yl_issue(b->yl_tables.p, b->yl_launches, b->subset_begin[2 * sub + 1], b->subset_begin[2 * sub + 3], st);
     }
 }
 
@@ -2222,6 +2275,7 @@ void op_batch_osem(et_batch *b, float *const *act, const float *const *sino, con
     const std::vector<const void *> key = batch_key(b, S, eps);
     if (key != b->key || b->subset_begin.empty()) {
         batch_build(b, S, st);
+        batch_streams(b, (int)b->yl_group_begin.size() - 1);
         b->key = key;
     }
     if (use_graph && (!b->exec || b->graph_stream != st)) {
