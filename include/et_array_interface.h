@@ -210,8 +210,11 @@ int et_array_osem(float *activity_data, unsigned int size_x, unsigned int size_y
  * `iterations` MLEM updates; et_array_osem: `iterations` et_array_osem_step
  * updates on subsets drawn from rand()) but run device-resident: data,
  * attenuation and iterate are uploaded once, one plan serves every step and
- * the MLEM sensitivity is computed once.  They return the first non-zero
- * status (the reference adds the statuses up).
+ * the MLEM sensitivity is computed once.  et_array_osem draws all its
+ * subsets from rand() before any device work (the GPU runtime's start-up may
+ * call rand() itself); the reference CPU path draws the same sequence step by
+ * step.  They return the first non-zero status (the reference adds the
+ * statuses up).
  *
  * Deterministic ordered subsets (no counterpart in the reference; the
  * reconstruction-driver row of the reference's rec_spect tool,

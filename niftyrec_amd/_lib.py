@@ -126,11 +126,21 @@ def bind(path):
 
 
 def load():
-    """The product library; raises when it has not been built."""
+    """The product library; raises when it has not been built.
+
+    torch (when installed) is imported first: its ROCm wheel carries its own
+    HIP runtime under the same soname as the system one the library links, and
+    whichever loads first serves the whole process.  Loaded in this order, the
+    library and torch share torch's runtime; the other order leaves torch
+    without a device."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("%s is not built: run `python -m niftyrec_amd.build`" % LIB_PATH)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         _lib = bind(LIB_PATH)
     return _lib
 

@@ -77,7 +77,26 @@ def build(verbose=True):
             print("[build]", " ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         os.replace(out_tmp, LIB_PATH)
+    build_tools(verbose)
     return LIB_PATH
+
+
+APPS = os.path.join(HERE, "apps")
+REC_SPECT = os.path.join(HERE, "rec_spect")
+
+
+def build_tools(verbose=True):
+    """The command line tool rec_spect (host C++, links the library in place:
+    its run path is the directory it sits in)."""
+    out_tmp = REC_SPECT + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", os.path.join(APPS, "rec_spect.cpp"),
+           os.path.join(APPS, "nifti_io.cpp"), "-o", out_tmp, "-L" + HERE, "-l:" + LIB_NAME,
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib", "-lz"]
+    if verbose:
+        print("[build]", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    os.replace(out_tmp, REC_SPECT)
+    return REC_SPECT
 
 
 if __name__ == "__main__":

@@ -1927,6 +1927,13 @@ int em_driver_host(int kind, float *activity, int size_x, int size_y, const floa
     if (size_x > ET_MAX_NX) return niftyrec_error_parameters;
     if (kind == EM_OSEM_ORDERED && (param <= 0 || param > V)) return niftyrec_error_parameters;
     if (kind == EM_OSEM_RANDOM && (param <= 0 || V / param <= 0)) return niftyrec_error_parameters;
+    // The random subsets are drawn before any device work, so the draws follow
+    // the caller's rand() state alone (the GPU runtime's start-up can call
+    // rand() itself).  The reference's CPU path draws them step by step with no
+    // other rand() call in between: the same sequence.
+    std::vector<std::vector<int>> draws;
+    if (kind == EM_OSEM_RANDOM)
+        for (int it = 0; it < iterations; it++) draws.push_back(random_subset(V, V / param));
     int st = select_device();
     if (st) return st;
     return guarded([&]() -> int {
@@ -1980,7 +1987,7 @@ int em_driver_host(int kind, float *activity, int size_x, int size_y, const floa
             float *d_msub = hc.alloc(P * nsub);
             int *d_idx = reinterpret_cast<int *>(hc.alloc(nsub));
             for (int it = 0; it < iterations; it++) {
-                const std::vector<int> idx = random_subset(V, nsub);
+                const std::vector<int> &idx = draws[it];
                 const std::vector<float> c = camera_subset(cams3, V, idx);
                 Cameras cs;
                 int rc = expand_cameras(c.data(), nsub, 3, vs, cs);
