@@ -267,7 +267,7 @@ struct et_plan {
     // rotated plane fits the LDS of one block; rows = yu = 16 * yl_rpl
     bool yl = false;
     int yl_rpl = 0, yl_nr = 0, yl_w = 0, yl_nbuf = 0;  // rows per lane, column rounds, columns per round, LDS planes
-    size_t yl_lds = 0;                                 // LDS bytes of a sweep block
+    size_t yl_lds = 0, yl_lds_bpu = 0;                 // LDS bytes of a projector / backprojection block
     DevBuf osem_tables;                                // argument records and block tables of the OSEM launches
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
     DevBuf fp_dev;                  // fingerprint (8 bytes)
@@ -355,6 +355,7 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
                 p->yl_w = (((nx + nr - 1) / nr) + 3) / 4 * 4;
                 p->yl_nbuf = nbuf;
                 p->yl_lds = p->R == 0 ? 0 : nbuf * plane;
+                p->yl_lds_bpu = p->R == 0 ? 0 : plane;
             }
         }
     }
@@ -1178,7 +1179,7 @@ struct YlTable {
                 int gx = 0, gy = 0;
                 args.push_back(fill(i, gx, gy, bytes));
                 bt.add(gx, gy);
-                lds = std::max(lds, plans[i]->yl_lds);
+                lds = std::max(lds, kind == YL_BPU ? plans[i]->yl_lds_bpu : plans[i]->yl_lds);
             }
             const auto &k = g.first;
             L.push_back({kind, std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k), lds, tb.put(args),

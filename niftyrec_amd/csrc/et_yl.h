@@ -14,11 +14,13 @@
 // (row) pass goes through LDS.  A sweep block owns one (view, depth chunk) and
 // all nx columns of each plane (NR rounds of W = blockDim / 16 columns), so the
 // row pass needs no halo recomputation: the LDS plane keeps R zero columns on
-// either side.  With two LDS planes alternating between depth planes, one
-// block barrier per plane orders every write after the reads of the same
+// either side.  The projector alternates two LDS planes between depth planes:
+// one block barrier per plane orders every write after the reads of the same
 // buffer (a wave reaches plane z + 2's write only after every wave has passed
 // plane z + 1's barrier, which follows its plane-z reads); with one plane
-// (a.yl_nbuf == 1) a second barrier follows the reads.
+// (a.yl_nbuf == 1, planes too large for two) a second barrier follows the
+// reads.  The backprojection always uses one plane and two barriers, so that
+// two of its blocks fit a CU.
 //
 // Layouts (rows = 16 * RPL, y fastest everywhere):
 //   factors   att[v][z'][x'][rows]  e^{-A(x', y, z')}, A the inclusive sum of mu'
@@ -372,7 +374,9 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 // Rotated-frame backprojection with cached attenuation factors:
 // u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ), s the ratio [v][x'][rows]
 // (ET_SRC_SINO) or 1 on the rows below ny (ET_SRC_ONES).  Reference
-// emission-lib/_et.cpp:441-471 (line backprojection, then the PSF).
+// emission-lib/_et.cpp:441-471 (line backprojection, then the PSF).  One LDS
+// plane and two barriers per plane: the small footprint (one plane, no
+// registers held across planes) lets two blocks share a CU.
 // ------------------------------------------------------------------------------
 template <int R, int RPL, int NR>
 __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
@@ -385,7 +389,7 @@ __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
     constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
     const int row0 = RPL * lane;
     const int PF = yl_plane_floats(a.nx, R, rows);
-    if constexpr (R > 0) yl_zero_lds(lds, a.yl_nbuf * PF);
+    if constexpr (R > 0) yl_zero_lds(lds, PF);
 
     float sv[NR][RPL];
 #pragma unroll
@@ -400,37 +404,20 @@ __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
         }
     }
     float *uv = a.u + (size_t)v * a.ustride + row0;
-    // attenuation factors one plane ahead (their latency passes under the
-    // previous plane's row pass and stores)
-    auto load_e = [&](int z, float (&e)[NR][RPL]) {
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-            const int xc = r * W + slot;
-            if (xc < a.nx && z < ze) yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
-        }
-    };
-    float en[NR][RPL];
-    load_e(zb, en);
     for (int z = zb; z < ze; z++) {
         const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
-        float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
-        float ec[NR][RPL];
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-#pragma unroll
-            for (int i = 0; i < RPL; i++) ec[r][i] = en[r][i];
-        load_e(z + 1, en);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
             const int xc = r * W + slot;
             if (xc < a.nx) {
-                float w[RPL];
+                float w[RPL], e[RPL];
+                yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e);
 #pragma unroll
-                for (int i = 0; i < RPL; i++) w[i] = sv[r][i] * ec[r][i];
+                for (int i = 0; i < RPL; i++) w[i] = sv[r][i] * e[i];
                 if constexpr (R > 0) {
                     float cp[RPL];
                     yl_column_pass<R, RPL>(w, kz + K, cp);
-                    lds_store<RPL>(plane + (xc + R) * CS + row0, cp);
+                    lds_store<RPL>(lds + (xc + R) * CS + row0, cp);
                 } else {
                     yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), w);
                 }
@@ -443,11 +430,11 @@ __device__ __forceinline__ void yl_bpu_body(const EtSweepArgs &a, const Blk &b)
                 const int xc = r * W + slot;
                 if (xc < a.nx) {
                     float o[RPL];
-                    yl_row_pass<R, RPL>(plane, CS, xc, row0, kz, o);
+                    yl_row_pass<R, RPL>(lds, CS, xc, row0, kz, o);
                     yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), o);
                 }
             }
-            if (a.yl_nbuf == 1) __syncthreads();
+            __syncthreads();  // the next plane's writes wait for these reads
         }
     }
 }
@@ -463,8 +450,10 @@ __global__ void __launch_bounds__(1024) k_yl_project(const EtSweepArgs *__restri
     yl_project_body<R, RPL, NR>(a, b);
 }
 
+// Two blocks per CU (at most 64 VGPRs; one LDS plane each): the barriers of
+// one block's planes overlap the other's gathers and stores.
 template <int R, int RPL, int NR>
-__global__ void __launch_bounds__(1024) k_yl_bpu(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
+__global__ void __launch_bounds__(1024, 2) k_yl_bpu(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
                                                  int n)
 {
     Blk b;
