@@ -268,6 +268,7 @@ struct et_plan {
     bool yl = false;
     int yl_rpl = 0, yl_nr = 0, yl_w = 0, yl_nbuf = 0;  // rows per lane, column rounds, columns per round, LDS planes
     size_t yl_lds = 0, yl_lds_bpu = 0;                 // LDS bytes of a projector / backprojection block
+    int yl_tiles = 1;                                  // projector column tiles (> 1: the tiled projector)
     DevBuf osem_tables;                                // argument records and block tables of the OSEM launches
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
     DevBuf fp_dev;                  // fingerprint (8 bytes)
@@ -356,6 +357,17 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
                 p->yl_nbuf = nbuf;
                 p->yl_lds = p->R == 0 ? 0 : nbuf * plane;
                 p->yl_lds_bpu = p->R == 0 ? 0 : plane;
+                // tiled projector where the volume is wider than one tile and
+                // two tiled blocks fit a CU's LDS (the point of tiling)
+                const size_t tl = et_yl_tile_lds_bytes(p->R, rows32);
+                int per_cu = 0;
+                if (hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) !=
+                    hipSuccess)
+                    per_cu = 0;
+                if (p->R > 0 && nx > ET_YL_TILE && 2 * tl <= (size_t)per_cu) {
+                    p->yl_tiles = (nx + ET_YL_TILE - 1) / ET_YL_TILE;
+                    p->yl_lds = tl;
+                }
             }
         }
     }
@@ -1155,6 +1167,7 @@ EtSweepArgs yl_sweep_args(const et_plan *p, const float *recs, int nv, int D, co
     a.zlen = zlen;
     a.att = const_cast<float *>(att);
     a.yl_nbuf = p->yl_nbuf;
+    a.yl_tiles = p->yl_tiles;
     return a;
 }
 
@@ -1192,20 +1205,25 @@ struct YlTable {
         return std::make_tuple(p->R, p->yl_rpl, p->yl_nr, 16 * p->yl_w);
     }
     static std::tuple<int, int, int, int> pull_key(const et_plan *p) { return std::make_tuple(0, p->yl_rpl, 0, 256); }
+    // projector launches: the tiled kernel is keyed with NR = 0
+    static std::tuple<int, int, int, int> project_key(const et_plan *p)
+    {
+        return p->yl_tiles > 1 ? std::make_tuple(p->R, p->yl_rpl, 0, 512) : sweep_key(p);
+    }
 
     // Projection of each plan's views [v0, v0 + nv) of its OSEM records into its
     // partial sums (depth chunks Dp of the plan).
     void project(const std::vector<et_plan *> &plans, const std::vector<int> &v0, const std::vector<int> &nv,
                  const std::vector<int> &Dp)
     {
-        grouped<EtSweepArgs>(YL_PROJECT, plans, sweep_key, [&](int i, int &gx, int &gy, double &bytes) {
+        grouped<EtSweepArgs>(YL_PROJECT, plans, project_key, [&](int i, int &gx, int &gy, double &bytes) {
             et_plan *p = plans[i];
             bytes += (double)nv[i] * (8.0 * p->nvox + 4.0 * p->P);  // activity and mu (factors) per view, sinogram
             EtSweepArgs a = yl_sweep_args(p, p->osem_recs.p + (size_t)v0[i] * ET_VIEW_REC, nv[i], Dp[i],
                                           p->osem_E.p + (size_t)v0[i] * p->nz * yl_plane_elems(p));
             a.act = p->x_yf.p;
             a.part = p->part.p;
-            gx = a.D;
+            gx = a.D * p->yl_tiles;
             gy = nv[i];
             return a;
         });
@@ -1282,7 +1300,7 @@ void yl_issue(const float *dev, const std::vector<YlLaunch> &L, size_t from, siz
         case YL_PROJECT:
         case YL_BPU: {
             ProfScope ps(prof, l.kind == YL_PROJECT ? ET_PROF_PROJECT : ET_PROF_BPU, l.bytes);
-            check_launch(et_launch_yl_sweep(l.kind == YL_PROJECT ? 0 : 1, l.R, l.RPL, l.NR, l.nthr, l.lds,
+            check_launch(et_launch_yl_sweep(l.kind == YL_BPU ? 1 : (l.NR == 0 ? 2 : 0), l.R, l.RPL, l.NR, l.nthr, l.lds,
                                             static_cast<const EtSweepArgs *>(args), tab, l.n, l.blocks, s));
             break;
         }

@@ -77,6 +77,7 @@ struct EtSweepArgs {
                              // nthr = roundup(nx, 64) (the block width); see et_launch_sweep's has_mu
                              // (row-lane kernels: [V][nz][nx][yu])
     int yl_nbuf;             // row-lane sweeps: LDS planes (2: alternate, one barrier per plane; 1: two)
+    int yl_tiles;            // row-lane projector: column tiles per plane (1: the whole width per block)
 };
 
 enum EtEpilogue {
@@ -175,8 +176,17 @@ int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_
 // Row-lane kernels of the OSEM subset loop (et_yl.h).  Grouped launches only
 // (tab as et_launch_sweep_group): sweeps kind 0 projection, 1 rotated-frame
 // backprojection, gx = depth chunks, gy = views, nthr = 16 * W threads, `lds`
-// bytes of LDS; pull gx = ceil(nx * nz / 16), gy = 1, 256 threads.  RPL
-// (rows per lane) in {2, 4, 6, 8}, NR (column rounds) in {1, 2, 3}.
+// bytes of LDS; kind 2 the tiled projection (R > 0), gx = depth chunks x
+// tiles (tile fastest), 512 threads, NR unused; pull gx = ceil(nx * nz / 16),
+// gy = 1, 256 threads.  RPL (rows per lane) in {2, 4, 6, 8}, NR (column
+// rounds) in {1, 2, 3}.
+// Tiled row-lane projector: columns per tile, LDS bytes of a block (two
+// planes of 64 + 2R columns of rows + 4 floats).
+#define ET_YL_TILE 64
+inline constexpr size_t et_yl_tile_lds_bytes(int R, int rows)
+{
+    return (size_t)2 * (ET_YL_TILE + 2 * R) * (rows + 4) * sizeof(float);
+}
 int et_launch_yl_sweep(int kind, int R, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args,
                        const void *tab, int n, int blocks, et_stream_t stream);
 int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void *tab, int n, int blocks,

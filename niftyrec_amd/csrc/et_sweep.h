@@ -576,6 +576,19 @@ template <int R>
 static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args, const int4 *tab,
                        int n, int blocks, hipStream_t st)
 {
+    if (kind == 2) {  // tiled projector (R > 0)
+#define ET_YLT(rpl)                                                                                               \
+        if (RPL == rpl) {                                                                                         \
+            allow_lds(k_yl_project_tile<R, rpl>, lds);                                                            \
+            hipLaunchKernelGGL((k_yl_project_tile<R, rpl>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);     \
+            return (int)hipGetLastError();                                                                        \
+        }
+        if constexpr (R > 0) {
+            ET_YLT(2) ET_YLT(4) ET_YLT(6) ET_YLT(8)
+        }
+#undef ET_YLT
+        return (int)hipErrorInvalidValue;
+    }
 #define ET_YL(rpl, nr)                                                                                            \
     if (RPL == rpl && NR == nr) {                                                                                 \
         if (kind == 0) {                                                                                          \

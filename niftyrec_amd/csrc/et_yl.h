@@ -371,6 +371,114 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 }
 
 // ------------------------------------------------------------------------------
+// Tiled projection: as yl_project_body, but block (tile, d, v) of 512 threads
+// (32 lane rows) owns YL_TILE = 64 columns of each plane, two per row, and
+// also computes the R columns on either side of its tile (a third round of
+// 2R rows) so that its x' pass needs no other block's columns.  Two LDS planes
+// of (64 + 2R) columns alternate as in the full-width projector; a block then
+// needs under half the LDS and two blocks share a CU, so one block's barriers
+// and gathers overlap the other's.  Each column's arithmetic is the full-width
+// kernel's, in the same order: the results are identical.
+// ------------------------------------------------------------------------------
+#define YL_TILE ET_YL_TILE
+
+template <int R, int RPL>
+__device__ __forceinline__ void yl_project_tile_body(const EtSweepArgs &a, const Blk &b)
+{
+    constexpr int K = 2 * R + 1, W = 32, NR = 2, NRR = NR + 1;
+    static_assert(W * NR == YL_TILE && 2 * R <= W, "tile shape");
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int T = a.yl_tiles, tile = b.x % T, d = b.x / T, v = b.y;
+    const int x_lo = tile * YL_TILE;
+    const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
+    constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
+    constexpr int PF = (YL_TILE + 2 * R) * CS;     // one LDS plane: LDS column c holds column x_lo - R + c
+    const int row0 = RPL * lane;
+    const int dx = a.yp, dz = a.yp * a.xp;
+    const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
+    yl_zero_lds(lds, 2 * PF);
+    const float *act = a.act + a.gy + row0;
+    // column of this slot in round r: owned columns in rounds 0 and 1, the
+    // tile's halo columns in round 2 (slots 0 .. 2R - 1); -1 where none
+    auto col = [&](int r) -> int {
+        if (r < NR) return x_lo + r * W + slot;
+        if (slot < R) return x_lo - R + slot;
+        return slot < 2 * R ? x_lo + YL_TILE + slot - R : -1;
+    };
+    auto valid = [&](int xc) { return xc >= 0 && xc < a.nx; };
+
+    float acc[NR][RPL], cmp[NR][RPL];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            acc[r][i] = 0.f;
+            cmp[r][i] = 0.f;
+        }
+    auto kahan = [&](int r, int i, float h, float e) {
+        const float y = fmaf(h, e, -cmp[r][i]);
+        const float t = acc[r][i] + y;
+        cmp[r][i] = (t - acc[r][i]) - y;
+        acc[r][i] = t;
+    };
+
+    constexpr int PB = 16 / NRR;  // planes per sample batch
+    YlSamp sb;
+    int kb = 0;
+    for (int z = zb; z < ze; z++, kb = kb + 1 == PB ? 0 : kb + 1) {
+        const float *kz = psf_plane<R>(a.psf, z);
+        if (kb == 0)  // (planes past the chunk and absent columns give samples no round takes)
+            sb = yl_samp_of(fast_sample(fv, (float)col(lane % NRR), z + lane / NRR, a.nx, a.nz, dx, dz));
+        // the owned columns' factors first (their latency passes under the gathers)
+        float e[NR][RPL];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = col(r);
+            if (valid(xc)) yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+        }
+        float *plane = lds + (z & 1) * PF;
+#pragma unroll
+        for (int r = 0; r < NRR; r++) {
+            const int xc = col(r);
+            if (valid(xc)) {
+                const Samp s = yl_samp_from(sb, kb * NRR + r);
+                float bl[RPL], cp[RPL];
+                yl_bilinear<RPL>(act + s.off, dx, dz, s, bl);
+                yl_column_pass<R, RPL>(bl, kz + K, cp);
+                lds_store<RPL>(plane + (xc - x_lo + R) * CS + row0, cp);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = col(r);
+            if (valid(xc)) {
+                float h[RPL];
+                yl_row_pass<R, RPL>(plane, CS, xc - x_lo, row0, kz, h);
+#pragma unroll
+                for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int xc = col(r);
+        if (!valid(xc)) continue;
+        if (d == a.D - 1 && a.bg_line != 0.f) {
+            float e[RPL];
+            yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + a.nz - 1) * a.nx + xc) * rows + row0, e);
+#pragma unroll
+            for (int i = 0; i < RPL; i++) kahan(r, i, a.bg_line, e[i]);
+        }
+        float out[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; i++) out[i] = acc[r][i] - cmp[r][i];
+        yl_store<RPL>(a.part + (((size_t)d * a.V + v) * a.nx + xc) * rows + row0, out);
+    }
+}
+
+// ------------------------------------------------------------------------------
 // Rotated-frame backprojection with cached attenuation factors:
 // u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ), s the ratio [v][x'][rows]
 // (ET_SRC_SINO) or 1 on the rows below ny (ET_SRC_ONES).  Reference
@@ -448,6 +556,17 @@ __global__ void __launch_bounds__(1024) k_yl_project(const EtSweepArgs *__restri
     if (!group_block(tab, n, b, j)) return;
     const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
     yl_project_body<R, RPL, NR>(a, b);
+}
+
+template <int R, int RPL>
+__global__ void __launch_bounds__(512, 2) k_yl_project_tile(const EtSweepArgs *__restrict__ args,
+                                                          const int4 *__restrict__ tab, int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
+    yl_project_tile_body<R, RPL>(a, b);
 }
 
 // Two blocks per CU (at most 64 VGPRs; one LDS plane each): the barriers of
