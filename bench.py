@@ -40,11 +40,15 @@ def parse():
 
 # Kernels of the profiled categories in the OSEM subset loop, as named in the
 # rocprofv3 summaries: the row-lane sweeps (PSF radius R, RPL rows per lane of
-# a 16-lane row holding a column's whole axial extent, NR adjacent columns per
-# row) and the pull with the OSEM update.
+# a 16-lane row holding a column's whole axial extent, NR column rounds), the
+# tiled projector where the volume is wider than one 64-column tile and two
+# tiled blocks fit a CU's 160 KB of LDS, and the pull with the OSEM update.
 def kernel_of(cat, R, ny, nx):
     rpl, nr = -(-ny // 32) * 2, -(-nx // 64)
-    return {"project": "void (anonymous namespace)::k_yl_project<%d, %d, %d>" % (R, rpl, nr),
+    tiled = R > 0 and nx > 64 and 2 * 2 * (64 + 2 * R) * (16 * rpl + 4) * 4 <= 160 * 1024
+    project = ("void (anonymous namespace)::k_yl_project_tile<%d, %d>" % (R, rpl) if tiled else
+               "void (anonymous namespace)::k_yl_project<%d, %d, %d>" % (R, rpl, nr))
+    return {"project": project,
             "bp_rotated_frame": "void (anonymous namespace)::k_yl_bpu<%d, %d, %d>" % (R, rpl, nr),
             "bp_pull": "void (anonymous namespace)::k_yl_pull<%d, 2>" % rpl}.get(cat)
 
@@ -72,12 +76,15 @@ def measured_traffic(prefix, grid=None):
     return None if best is None else best[1] * 1e6
 
 
-def sweep_grid(cat, nx, nz, views, depth_chunks):
+def sweep_grid(cat, nx, nz, views, depth_chunks, tiled=False):
     """Work-items of one subset launch: a sweep runs views x depth chunks blocks
-    of 16 W threads (W slots of at most 3 adjacent columns, a multiple of 4);
-    the pull ceil(nx nz / 16) blocks of 256."""
+    of 16 W threads (W slots of at most 3 column rounds, a multiple of 4), the
+    tiled projector views x depth chunks x ceil(nx / 64) blocks of 512; the
+    pull ceil(nx nz / 16) blocks of 256."""
     if cat == "bp_pull":
         return -(-nx * nz // 16) * 256
+    if tiled and cat == "project":
+        return views * depth_chunks * -(-nx // 64) * 512
     nr = -(-nx // 64)
     w = (-(-nx // nr) + 3) // 4 * 4
     return views * depth_chunks * 16 * w
@@ -381,6 +388,7 @@ def main():
     P = nx * ny
     bytes_iter = V * (16.0 * n + 8.0 * P) + S * 16.0 * n  # per view: act+mu read, sino read+write; per subset: x, sens read, x write
     launches, tot_ms, tot_bytes = prof[cat]
+    kname = kernel_of(cat, (psf.shape[-1] - 1) // 2, ny, nx)
     achieved = tot_bytes / (tot_ms * 1e-3)
     line = {
         "metric": METRIC,
@@ -403,8 +411,10 @@ def main():
         "hbm_frac_of_peak": bytes_iter / (ms * 1e-3) / HBM_PEAK,
         "roofline": {"bound": "hbm", "kernel": cat, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": measured_traffic(kernel_of(cat, (psf.shape[-1] - 1) // 2, ny, nx),
-                                                 sweep_grid(cat, nx, nz, V // S, dchunks[cat == "bp_rotated_frame"])),
+                     "traffic": measured_traffic(kname, sweep_grid(cat, nx, nz, V // S,
+                                                                   dchunks[cat == "bp_rotated_frame"],
+                                                                   "project_tile" in (kname or ""))),
+                     "kernel_name": kname,
                      "avg_launch_ms": tot_ms / launches, "algorithmic_bytes_per_launch": tot_bytes / launches,
                      "depth_chunks": dchunks[cat == "bp_rotated_frame"]},
         "kernel_ms_per_step": {k: v[1] / n_split for k, v in split.items() if v[0]},

@@ -139,19 +139,20 @@ def test_no_device_is_a_status_not_an_exit():
 def test_bench_finds_its_traffic_rows():
     """bench.py reads the measured HBM traffic of the dominant kernel from the
     committed rocprofv3 summary: the row of the subset launch grid (15 views x
-    16 depth chunks blocks of 1024 threads at 128^3, the chunking the profiler
-    runs fix) must be there for the row-lane projector and backprojection, and
-    the pull's row (1024 blocks of 256)."""
+    16 depth chunks x 2 tiles blocks of 512 threads for the tiled projector,
+    15 x 16 blocks of 1024 for the backprojection at 128^3, the chunking the
+    profiler runs fix) must be there, and the pull's row (1024 blocks of 256)."""
     import importlib.util
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
     for cat in ("project", "bp_rotated_frame", "bp_pull"):
-        grid = bench.sweep_grid(cat, 128, 128, 15, 16)
+        grid = bench.sweep_grid(cat, 128, 128, 15, 16, tiled=True)
         assert grid == (262144 if cat == "bp_pull" else 15 * 16 * 1024)
         t = bench.measured_traffic(bench.kernel_of(cat, 5, 128, 128), grid)
         assert t is not None and 1e8 < t < 1e9, (cat, t)
-    assert bench.kernel_of("project", 5, 128, 128) == "void (anonymous namespace)::k_yl_project<5, 8, 2>"
+    assert bench.kernel_of("project", 5, 128, 128) == "void (anonymous namespace)::k_yl_project_tile<5, 8>"
+    assert bench.kernel_of("project", 7, 128, 128) == "void (anonymous namespace)::k_yl_project<7, 8, 2>"
 
 
 def _no_gpu(L):
