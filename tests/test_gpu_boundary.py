@@ -85,7 +85,8 @@ def test_by_value_spellings_reference_conventions():
                             None, zeros3.ctypes.data_as(ip), None, zeros3.ctypes.data_as(ip), np.float32(0.0),
                             np.float32(0.0), np.int32(1), np.int32(1))
     assert st == 0
-    report_parity("et_array_project (no psf, no attenuation)", proj2.reshape(V, ny, nx), orc.project(act, None, cams))
+    report_parity("et_array_project (no psf, no attenuation)", proj2.reshape(V, ny, nx), orc.project(act, None, cams),
+                  orc.project(act, None, cams, dtype=np.float64))
 
 
 def test_niftyrec_module_project_backproject():
@@ -111,19 +112,20 @@ def test_niftyrec_module_project_backproject():
     assert n >= 1 and len(info) == NR.SIZE_INFO * min(n, NR.MAX_DEVICES)
 
 
-def _osmapem_oracle(subset_order, x, sino, cams, mu, psf, r, beta=0.0, prior=0.0, eps=1e-6):
-    """NiftyRec.py:224-246 with the oracle as projector (float32 numpy steps)."""
+def _osmapem_oracle(subset_order, x, sino, cams, mu, psf, r, beta=0.0, prior=0.0, eps=1e-6, dtype=np.float32):
+    """NiftyRec.py:224-246 with the oracle as projector (numpy steps in dtype)."""
+    x = np.asarray(x, dtype)
     N1, N2, N3 = x.shape
     n_cam = cams.shape[1]
     n_sub = int(np.int32(np.round(n_cam / subset_order)))
     idx = np.int32(np.round(0.5 + (n_cam - 0.5) * r) - 1)
     cs = cams[:, idx]
-    norm = orc.backproject(np.ones((n_sub, N2, N3), np.float32), mu, cs, psf)
+    norm = orc.backproject(np.ones((n_sub, N2, N3), dtype), mu, cs, psf, dtype=dtype)
     norm = norm - beta * prior
-    norm = norm * (norm > 0) + np.float32(eps)
-    proj = orc.project(x, mu, cs, psf)
-    proj = proj * (proj > 0) + np.float32(eps)
-    upd = orc.backproject(np.asarray(sino)[idx] / proj, mu, cs, psf)
+    norm = norm * (norm > 0) + dtype(eps)
+    proj = orc.project(x, mu, cs, psf, dtype=dtype)
+    proj = proj * (proj > 0) + dtype(eps)
+    upd = orc.backproject(np.asarray(sino, dtype)[idx] / proj, mu, cs, psf, dtype=dtype)
     upd = upd * (upd > 0)
     return x * upd / norm
 
@@ -139,7 +141,8 @@ def test_niftyrec_osmapem_step_and_reconstructor():
     x0 = np.ones_like(act)
     r = np.random.default_rng(43).random(4)
     got = NR.et_osmapem_step(4, x0, y, cams, mu, psf.T, 0.0, 0.0, 1, rng=np.random.default_rng(43))
-    report_parity("NiftyRec.et_osmapem_step", got, _osmapem_oracle(4, x0, y, cams, mu, psf, r))
+    report_parity("NiftyRec.et_osmapem_step", got, _osmapem_oracle(4, x0, y, cams, mu, psf, r),
+                  _osmapem_oracle(4, x0, y, cams, mu, psf, r, dtype=np.float64))
 
     rec = NR.Reconstructor(act.shape)
     rec.set_cameras_matrix(cams)
@@ -149,10 +152,12 @@ def test_niftyrec_osmapem_step_and_reconstructor():
     np.random.seed(44)
     assert rec.reconstruct("osem", {"steps": 2, "subset_order": 4}, block=True)
     np.random.seed(44)
-    x = np.ones_like(act)
-    for _ in range(2):
-        x = _osmapem_oracle(4, x, y, cams, mu, psf, np.random.rand(4))
-    report_parity("NiftyRec.Reconstructor osem (2 steps)", rec.activity, x)
+    draws = [np.random.rand(4) for _ in range(2)]
+    x, x64 = np.ones_like(act), np.ones(act.shape)
+    for r in draws:
+        x = _osmapem_oracle(4, x, y, cams, mu, psf, r)
+        x64 = _osmapem_oracle(4, x64, y, cams, mu, psf, r, dtype=np.float64)
+    report_parity("NiftyRec.Reconstructor osem (2 steps)", rec.activity, x, x64)
 
 
 @pytest.fixture(scope="module")
@@ -193,11 +198,12 @@ def test_et_array_osem_random_driver(spect64):
     libc.srand(77)
     got = api.osem(y, 4, 0.0, 360.0 * (1 - 1.0 / V), 3, psf=psf, attenuation=mu)
     libc.srand(77)
-    x = np.ones_like(mu)
+    x, x64 = np.ones_like(mu), np.ones(mu.shape)
     for _ in range(3):
         idx = orc.osem_subset(V, V // 4)
         x = orc.mlem_step(x, y[idx], mu, cams3[:, idx], psf)
-    report_parity("et_array_osem (3 random-subset steps)", got, x)
+        x64 = orc.mlem_step(x64, y[idx], mu, cams3[:, idx], psf, dtype=np.float64)
+    report_parity("et_array_osem (3 random-subset steps)", got, x, x64)
 
 
 def test_et_array_osem_ordered_driver(spect64):
@@ -223,10 +229,11 @@ def test_driver_general_path_and_status():
     cams3 = api.make_cameras(0.0, 315.0, 8, 0)
     y = orc.project(act, mu, cams3, psf)
     got = api.osem_ordered(y, 2, 0.0, 315.0, 1, psf=psf, attenuation=mu, rotation_axis=0)
-    x = np.ones_like(act)
+    x, x64 = np.ones_like(act), np.ones(act.shape)
     for s in range(2):
         x = orc.mlem_step(x, y[s::2], mu, cams3[:, s::2], psf)
-    report_parity("et_array_osem_ordered about x (general path)", got, x)
+        x64 = orc.mlem_step(x64, y[s::2], mu, cams3[:, s::2], psf, dtype=np.float64)
+    report_parity("et_array_osem_ordered about x (general path)", got, x, x64)
     L = _lib.load()
     a = np.zeros_like(act)
     st = L.et_array_mlem(a.ctypes.data_as(_lib.fp), 24, 16, np.ascontiguousarray(y).ctypes.data_as(_lib.fp), 8,
@@ -251,28 +258,31 @@ def test_osem_cache_follows_the_attenuation_map():
         plan.osem(X, Y, M, subsets=3, iterations=1)
         return X.cpu().numpy()
 
-    def oracle(m):
-        x = np.ones_like(act)
+    def oracle(m, dtype=np.float32):
+        x = np.ones(act.shape, dtype)
         for s in range(3):
-            x = orc.mlem_step(x, y[s::3], m, cams[s::3], psf)
+            x = orc.mlem_step(x, y[s::3], m, cams[s::3], psf, dtype=dtype)
         return x
 
+    def refs(m):
+        return oracle(m), oracle(m, np.float64)
+
     M = torch.from_numpy(mu).cuda()
-    report_parity("osem, first map", run(M), oracle(mu))
+    report_parity("osem, first map", run(M), *refs(mu))
     M.mul_(0.5)
-    report_parity("osem, map halved in place", run(M), oracle(mu * np.float32(0.5)))
+    report_parity("osem, map halved in place", run(M), *refs(mu * np.float32(0.5)))
     addr = M.data_ptr()
     del M
     M2 = torch.empty(act.shape, device="cuda")
     print("second map at the first map's address: %s" % (M2.data_ptr() == addr))
     mu2 = (mu * np.float32(1.5)).astype(np.float32)
     M2.copy_(torch.from_numpy(mu2))
-    report_parity("osem, another map (same address: %s)" % (M2.data_ptr() == addr), run(M2), oracle(mu2))
+    report_parity("osem, another map (same address: %s)" % (M2.data_ptr() == addr), run(M2), *refs(mu2))
     plan.set_mu_check(False)
     run(M2)
     M2.mul_(0.25)
     plan.invalidate()
-    report_parity("osem, address key + invalidate", run(M2), oracle(mu2 * np.float32(0.25)))
+    report_parity("osem, address key + invalidate", run(M2), *refs(mu2 * np.float32(0.25)))
     # content key (the C library's default): a map rewritten without torch
     # knowing (through a second tensor on the same memory, whose version
     # counter is its own) is still caught
@@ -284,4 +294,4 @@ def test_osem_cache_follows_the_attenuation_map():
     alias.data.mul_(2.0)
     Xc = torch.ones(act.shape, device="cuda")
     pc.osem(Xc, Y, Mc, subsets=3, iterations=1)
-    report_parity("osem, content key, map rewritten behind torch", Xc.cpu().numpy(), oracle(mu * np.float32(2.0)))
+    report_parity("osem, content key, map rewritten behind torch", Xc.cpu().numpy(), *refs(mu * np.float32(2.0)))

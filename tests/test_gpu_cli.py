@@ -63,11 +63,12 @@ def test_rec_spect_ordered_osem(data):
     got = _rec(d, "osem.nii.gz", "-i", "2", "-u", "4", "-a", str(d / "mu.nii"))
     lib = api.osem_ordered(y, 4, 0.0, LAST, 2, psf=psf, attenuation=mu, epsilon=1e-7)
     assert np.array_equal(got, lib)
-    x = np.ones_like(mu)
+    x, x64 = np.ones_like(mu), np.ones(mu.shape)
     for _ in range(2):
         for s in range(4):
             x = orc.mlem_step(x, y[s::4], mu, cams3[:, s::4], psf, eps=1e-7)
-    report_parity("rec_spect ordered OSEM (4 subsets, 2 passes)", got, x)
+            x64 = orc.mlem_step(x64, y[s::4], mu, cams3[:, s::4], psf, eps=1e-7, dtype=np.float64)
+    report_parity("rec_spect ordered OSEM (4 subsets, 2 passes)", got, x, x64)
 
 
 def test_rec_spect_legacy_subsets_without_attenuation(data):

@@ -64,10 +64,12 @@ def test_project_odd_shapes(lib):
         act, mu, cams, psf = _case(shape=shape, V=5, K=3, seed=shape[1])
         st, got = refabi.project(lib, act, mu, cams, psf)
         assert st == 0
-        _report("project %s" % (shape,), got, orc.project(act, mu, cams, psf))
+        _report("project %s" % (shape,), got, orc.project(act, mu, cams, psf),
+                orc.project(act, mu, cams, psf, dtype=np.float64))
         st, bp = refabi.backproject(lib, got, mu, cams, psf)
         assert st == 0
-        _report("backproject %s" % (shape,), bp, orc.backproject(got, mu, cams, psf))
+        _report("backproject %s" % (shape,), bp, orc.backproject(got, mu, cams, psf),
+                orc.backproject(got, mu, cams, psf, dtype=np.float64))
 
 
 def test_background_nonzero(lib):
@@ -75,10 +77,12 @@ def test_background_nonzero(lib):
     act, mu, cams, psf = _case(seed=2)
     st, got = refabi.project(lib, act, mu, cams, psf, bg=0.5, bga=0.02)
     assert st == 0
-    _report("project bg", got, orc.project(act, mu, cams, psf, 0.5, 0.02))
+    _report("project bg", got, orc.project(act, mu, cams, psf, 0.5, 0.02),
+            orc.project(act, mu, cams, psf, 0.5, 0.02, dtype=np.float64))
     st, bp = refabi.backproject(lib, got, mu, cams, psf, bg=0.5, bga=0.02)
     assert st == 0
-    _report("backproject bg", bp, orc.backproject(got, mu, cams, psf, 0.5, 0.02))
+    _report("backproject bg", bp, orc.backproject(got, mu, cams, psf, 0.5, 0.02),
+            orc.backproject(got, mu, cams, psf, 0.5, 0.02, dtype=np.float64))
 
 
 def test_three_axis_cameras(lib):
@@ -88,10 +92,11 @@ def test_three_axis_cameras(lib):
     c3 = rng.uniform(-0.6, 0.6, (3, 6)).astype(np.float32)
     st, got = refabi.project(lib, act, mu, c3, psf)
     assert st == 0
-    _report("project 3-axis", got, orc.project(act, mu, c3, psf))
+    _report("project 3-axis", got, orc.project(act, mu, c3, psf), orc.project(act, mu, c3, psf, dtype=np.float64))
     st, bp = refabi.backproject(lib, got, mu, c3, psf)
     assert st == 0
-    _report("backproject 3-axis", bp, orc.backproject(got, mu, c3, psf))
+    _report("backproject 3-axis", bp, orc.backproject(got, mu, c3, psf),
+            orc.backproject(got, mu, c3, psf, dtype=np.float64))
 
 
 def test_wide_psf_general_path(lib):
@@ -99,10 +104,11 @@ def test_wide_psf_general_path(lib):
     act, mu, cams, psf = _case(shape=(32, 16, 32), V=4, K=25, seed=4)
     st, got = refabi.project(lib, act, mu, cams, psf)
     assert st == 0
-    _report("project K=25", got, orc.project(act, mu, cams, psf))
+    _report("project K=25", got, orc.project(act, mu, cams, psf), orc.project(act, mu, cams, psf, dtype=np.float64))
     st, bp = refabi.backproject(lib, got, mu, cams, psf)
     assert st == 0
-    _report("backproject K=25", bp, orc.backproject(got, mu, cams, psf))
+    _report("backproject K=25", bp, orc.backproject(got, mu, cams, psf),
+            orc.backproject(got, mu, cams, psf, dtype=np.float64))
 
 
 def test_attenuation_only_projection(lib):
@@ -110,7 +116,8 @@ def test_attenuation_only_projection(lib):
     act, mu, cams, psf = _case(seed=5)
     st, got = refabi.project(lib, None, mu, cams, None, bg=2.0)
     assert st == 0
-    _report("project mu-only", got, orc.project(None, mu, cams, None, 2.0))
+    _report("project mu-only", got, orc.project(None, mu, cams, None, 2.0),
+            orc.project(None, mu, cams, None, 2.0, dtype=np.float64))
 
 
 @pytest.mark.parametrize("K", [0, 5])
@@ -160,7 +167,7 @@ def test_osem_step_random_subset(lib):
     libc.srand(1234)
     idx = orc.osem_subset(len(cams), len(cams) // 4)
     ref = orc.mlem_step(np.ones_like(act), y[idx], mu, cams[idx], psf)
-    _report("osem step", got, ref)
+    _report("osem step", got, ref, orc.mlem_step(np.ones(act.shape), y[idx], mu, cams[idx], psf, dtype=np.float64))
 
 
 def test_rotate(lib):
@@ -171,7 +178,8 @@ def test_rotate(lib):
         ang = rng.uniform(-1, 1, 3).astype(np.float32)
         st, got = refabi.rotate(lib, img, ang, ctr, 0.25, order)
         assert st == 0
-        _report("rotate order %d" % order, got, orc.resample(img, orc.rotation_matrix(ang, ctr, order), 0.25))
+        _report("rotate order %d" % order, got, orc.resample(img, orc.rotation_matrix(ang, ctr, order), 0.25),
+                orc.resample(img, orc.rotation_matrix(ang, ctr, order, dtype=np.float64), 0.25, dtype=np.float64))
 
 
 def test_affine(lib):
@@ -221,7 +229,8 @@ def test_spect64_full_size(lib):
     _report("spect64 project", got, orc.project(act, mu, cams, psf), orc.project(act, mu, cams, psf, dtype=np.float64))
     st, bp = refabi.backproject(lib, got, mu, cams, psf)
     assert st == 0
-    _report("spect64 backproject", bp, orc.backproject(got, mu, cams, psf))
+    _report("spect64 backproject", bp, orc.backproject(got, mu, cams, psf),
+            orc.backproject(got, mu, cams, psf, dtype=np.float64))
 
 
 def test_spect256_full_size_views():
@@ -237,7 +246,8 @@ def test_spect256_full_size_views():
     plan = Plan(act.shape, cams, psf)
     A, M = torch.from_numpy(act).cuda(), torch.from_numpy(mu).cuda()
     p = plan.project(A, M).cpu().numpy()
-    _report("spect256 project (4 views)", p, orc.project(act, mu, cams, psf))
+    _report("spect256 project (4 views)", p, orc.project(act, mu, cams, psf),
+            orc.project(act, mu, cams, psf, dtype=np.float64))
     b = plan.backproject(torch.from_numpy(p).cuda(), M).cpu().numpy()
     _report("spect256 backproject (4 views)", b, orc.backproject(p, mu, cams, psf),
             orc.backproject(p, mu, cams, psf, dtype=np.float64))
@@ -270,5 +280,6 @@ def test_project_partial_attenuation_only():
     _, mu, cams, _ = _case(shape=(16, 10, 16), V=3, K=0, seed=32)
     sino, part = api.project_partial(None, cams, None, mu, background=1.5)
     rs, rp = orc.project_partial(None, mu, cams, None, bg=1.5)
-    _report("partial (attenuation only) sinogram", sino, rs)
-    _report("partial (attenuation only) sums", part, rp)
+    rs64, rp64 = orc.project_partial(None, mu, cams, None, bg=1.5, dtype=np.float64)
+    _report("partial (attenuation only) sinogram", sino, rs, rs64)
+    _report("partial (attenuation only) sums", part, rp, rp64)
