@@ -389,7 +389,11 @@ __device__ __forceinline__ void yl_project_tile_body(const EtSweepArgs &a, const
     static_assert(W * NR == YL_TILE && 2 * R <= W, "tile shape");
     extern __shared__ float lds[];
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    const int T = a.yl_tiles, tile = b.x % T, d = b.x / T, v = b.y;
+    // the T tiles of one (view, depth chunk) on one XCD, dispatched together:
+    // each tile's halo columns are the neighbour's owned columns, in the same L2
+    const int T = a.yl_tiles;
+    const int2 o = xcd_order(b, T, a.D * a.V, b.x % T, b.x / T + a.D * b.y);
+    const int tile = o.x, d = o.y % a.D, v = o.y / a.D;
     const int x_lo = tile * YL_TILE;
     const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
     constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
@@ -443,10 +447,9 @@ __device__ __forceinline__ void yl_project_tile_body(const EtSweepArgs &a, const
             const int xc = col(r);
             if (valid(xc)) {
                 const Samp s = yl_samp_from(sb, kb * NRR + r);
-                float bl[RPL], cp[RPL];
+                float bl[RPL];
                 yl_bilinear<RPL>(act + s.off, dx, dz, s, bl);
-                yl_column_pass<R, RPL>(bl, kz + K, cp);
-                lds_store<RPL>(plane + (xc - x_lo + R) * CS + row0, cp);
+                lds_store<RPL>(plane + (xc - x_lo + R) * CS + row0, bl);
             }
         }
         __syncthreads();
@@ -454,8 +457,9 @@ __device__ __forceinline__ void yl_project_tile_body(const EtSweepArgs &a, const
         for (int r = 0; r < NR; r++) {
             const int xc = col(r);
             if (valid(xc)) {
-                float h[RPL];
-                yl_row_pass<R, RPL>(plane, CS, xc - x_lo, row0, kz, h);
+                float t[RPL], h[RPL];
+                yl_row_pass<R, RPL>(plane, CS, xc - x_lo, row0, kz, t);
+                yl_column_pass<R, RPL>(t, kz + K, h);
 #pragma unroll
                 for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
             }
