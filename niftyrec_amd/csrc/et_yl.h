@@ -562,8 +562,10 @@ __global__ void __launch_bounds__(1024) k_yl_project(const EtSweepArgs *__restri
     yl_project_body<R, RPL, NR>(a, b);
 }
 
+// Two blocks per CU: the second launch-bounds argument is the minimum number
+// of waves per SIMD (4: at most 128 VGPRs).
 template <int R, int RPL>
-__global__ void __launch_bounds__(512, 2) k_yl_project_tile(const EtSweepArgs *__restrict__ args,
+__global__ void __launch_bounds__(512, 4) k_yl_project_tile(const EtSweepArgs *__restrict__ args,
                                                           const int4 *__restrict__ tab, int n)
 {
     Blk b;
@@ -573,10 +575,13 @@ __global__ void __launch_bounds__(512, 2) k_yl_project_tile(const EtSweepArgs *_
     yl_project_tile_body<R, RPL>(a, b);
 }
 
-// Two blocks per CU (at most 64 VGPRs; one LDS plane each): the barriers of
-// one block's planes overlap the other's gathers and stores.
+// One LDS plane per block; where the kernel needs at most 64 VGPRs (8 waves
+// per SIMD: every instantiation with NR <= 2, such as 54 at R = 5, RPL = 8),
+// two blocks share a CU and the barriers of one block's planes overlap the
+// other's gathers and stores.  Not forced by the launch bounds: the widest
+// instantiations (NR = 3, R >= 7) would spill.
 template <int R, int RPL, int NR>
-__global__ void __launch_bounds__(1024, 2) k_yl_bpu(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
+__global__ void __launch_bounds__(1024) k_yl_bpu(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
                                                  int n)
 {
     Blk b;
