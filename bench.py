@@ -41,11 +41,10 @@ def parse():
 # Kernels of the profiled categories in the OSEM subset loop, as named in the
 # rocprofv3 summaries: the row-lane sweeps (PSF radius R, RPL rows per lane of
 # a 16-lane row holding a column's whole axial extent, NR column rounds), the
-# tiled projector where the volume is wider than one 64-column tile and two
-# tiled blocks fit a CU's 160 KB of LDS, and the pull with the OSEM update.
-def kernel_of(cat, R, ny, nx):
+# tiled projector when the plan chose it (Plan.projector() > 1), and the pull
+# with the OSEM update.
+def kernel_of(cat, R, ny, nx, tiled):
     rpl, nr = -(-ny // 32) * 2, -(-nx // 64)
-    tiled = R > 0 and nx > 64 and 2 * 2 * (64 + 2 * R) * (16 * rpl + 4) * 4 <= 160 * 1024
     project = ("void (anonymous namespace)::k_yl_project_tile<%d, %d>" % (R, rpl) if tiled else
                "void (anonymous namespace)::k_yl_project<%d, %d, %d>" % (R, rpl, nr))
     return {"project": project,
@@ -388,7 +387,7 @@ def main():
     P = nx * ny
     bytes_iter = V * (16.0 * n + 8.0 * P) + S * 16.0 * n  # per view: act+mu read, sino read+write; per subset: x, sens read, x write
     launches, tot_ms, tot_bytes = prof[cat]
-    kname = kernel_of(cat, (psf.shape[-1] - 1) // 2, ny, nx)
+    kname = kernel_of(cat, (psf.shape[-1] - 1) // 2, ny, nx, plan.projector() > 1)
     achieved = tot_bytes / (tot_ms * 1e-3)
     line = {
         "metric": METRIC,

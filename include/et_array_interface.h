@@ -298,6 +298,12 @@ int et_plan_set_tuning(et_plan *plan, int depth_chunks, int view_batch);
 /* Depth chunks per ray of the cached OSEM data: projection and rotated-frame
  * backprojection sweeps (0 and 0 before the first et_plan_osem). */
 int et_plan_get_tuning(const et_plan *plan, int *depth_chunks_project, int *depth_chunks_backproject);
+/* Projector of the cached OSEM data: *tiles = the number of 64-column tiles of
+ * the tiled row-lane projector, 1 for the full-width row-lane projector, 0 for
+ * the slab kernels or before the first et_plan_osem.  The choice is timed
+ * with the depth chunking where both apply (an extension: no counterpart in
+ * the reference). */
+int et_plan_get_projector(const et_plan *plan, int *tiles);
 /* Kernel timing with HIP events on the plan's launches (category ids below):
  * et_plan_profile times every category (enable != 0) or none; _select times
  * the categories whose bits are set in `categories` (bit ET_PROF_*), so that

@@ -77,6 +77,7 @@ SIGNATURES = {
     "et_plan_invalidate": (c_int, [c_void_p]),
     "et_plan_get_device": (c_int, [c_void_p, P(c_int)]),
     "et_plan_get_tuning": (c_int, [c_void_p, P(c_int), P(c_int)]),
+    "et_plan_get_projector": (c_int, [c_void_p, P(c_int)]),
     "et_plan_profile_select": (c_int, [c_void_p, c_uint]),
     "et_plan_profile_sample": (c_int, [c_void_p, c_int]),
     "et_plan_profile": (c_int, [c_void_p, c_int]),

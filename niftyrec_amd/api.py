@@ -455,6 +455,14 @@ class Plan:
         _lib.check(self._L.et_plan_get_tuning(self._h, ctypes.byref(dp), ctypes.byref(db)), "et_plan_get_tuning")
         return dp.value, db.value
 
+    def projector(self):
+        """Projector of the cached OSEM data: the number of 64-column tiles of
+        the tiled row-lane projector, 1 for the full-width row-lane projector,
+        0 for the slab kernels or before the first osem call."""
+        t = ctypes.c_int()
+        _lib.check(self._L.et_plan_get_projector(self._h, ctypes.byref(t)), "et_plan_get_projector")
+        return t.value
+
     def profile(self, enable=True, categories=None, every=1):
         """Time the plan's launches with HIP events: every category, or only the
         named ones (names of _lib.PROF_NAMES), and of those every `every`-th

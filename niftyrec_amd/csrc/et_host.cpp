@@ -269,6 +269,8 @@ struct et_plan {
     int yl_rpl = 0, yl_nr = 0, yl_w = 0, yl_nbuf = 0;  // rows per lane, column rounds, columns per round, LDS planes
     size_t yl_lds = 0, yl_lds_bpu = 0;                 // LDS bytes of a projector / backprojection block
     int yl_tiles = 1;                                  // projector column tiles (> 1: the tiled projector)
+    int yl_tile_n = 1;                                 // tiles of the tiled projector where it applies, else 1
+    size_t yl_lds_full = 0, yl_lds_tile = 0;           // LDS of a full-width / tiled projector block
     DevBuf osem_tables;                                // argument records and block tables of the OSEM launches
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
     DevBuf fp_dev;                  // fingerprint (8 bytes)
@@ -355,7 +357,7 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
                 p->yl_nr = nr;
                 p->yl_w = (((nx + nr - 1) / nr) + 3) / 4 * 4;
                 p->yl_nbuf = nbuf;
-                p->yl_lds = p->R == 0 ? 0 : nbuf * plane;
+                p->yl_lds = p->yl_lds_full = p->R == 0 ? 0 : nbuf * plane;
                 p->yl_lds_bpu = p->R == 0 ? 0 : plane;
                 // tiled projector where the volume is wider than one tile and
                 // two tiled blocks fit a CU's LDS (the point of tiling)
@@ -365,8 +367,8 @@ int build_plan(et_plan *p, int nx, int ny, int nz, const Cameras &cams, const fl
                     hipSuccess)
                     per_cu = 0;
                 if (p->R > 0 && nx > ET_YL_TILE && 2 * tl <= (size_t)per_cu) {
-                    p->yl_tiles = (nx + ET_YL_TILE - 1) / ET_YL_TILE;
-                    p->yl_lds = tl;
+                    p->yl_tile_n = p->yl_tiles = (nx + ET_YL_TILE - 1) / ET_YL_TILE;
+                    p->yl_lds = p->yl_lds_tile = tl;
                 }
             }
         }
@@ -1327,22 +1329,41 @@ void yl_upload(DevBuf &buf, const YlTable &t, hipStream_t st)
     check_hip(hipStreamSynchronize(st), niftyrec_error_transfergpu);
 }
 
+// Projector kind of a row-lane plan: the tiled kernel (where it applies) or the
+// full-width one.
+void yl_use_tiles(et_plan *p, bool tiled)
+{
+    tiled = tiled && p->yl_tile_n > 1;
+    p->yl_tiles = tiled ? p->yl_tile_n : 1;
+    p->yl_lds = tiled ? p->yl_lds_tile : p->yl_lds_full;
+}
+
 // Depth chunking of the row-lane projection (mode ET_MODE_PROJECT, timed with
 // the ratio that reads its partial sums) or backprojection (ET_MODE_BPU, timed
 // behind the chosen projection and ratio) on the first subset: candidates 8,
 // 16, 32 chunks and the count that puts one block per CU, three launches each
 // after a warm-up, in three interleaved rounds; the fastest round decides.
+// Where the tiled projector applies, the projection candidates are timed with
+// both projector kinds, and the plan keeps the kind of the fastest.
 int yl_tune_chunks(et_plan *p, int mode, int nv, int Dp = 0)
 {
     int cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
         cus = 256;
-    std::vector<int> cand;
+    std::vector<int> chunks;
     for (int D : {8, 16, 32, cus / std::max(nv, 1)}) {
         int Deff, zlen;
         yl_chunks(p, D, Deff, zlen);
-        if (Deff >= 1 && std::find(cand.begin(), cand.end(), Deff) == cand.end()) cand.push_back(Deff);
+        if (Deff >= 1 && std::find(chunks.begin(), chunks.end(), Deff) == chunks.end()) chunks.push_back(Deff);
+    }
+    // (depth chunks, tiled projector); the backprojection keeps the projector chosen
+    const bool both = mode == ET_MODE_PROJECT && p->yl_tile_n > 1;
+    const bool tiled_now = p->yl_tiles > 1;
+    std::vector<std::pair<int, bool>> cand;
+    for (int D : chunks) {
+        cand.push_back({D, mode == ET_MODE_PROJECT ? p->yl_tile_n > 1 : tiled_now});
+        if (both) cand.push_back({D, false});
     }
     const std::vector<et_plan *> one{p};
     const std::vector<int> v0{0}, nvv{nv};
@@ -1355,7 +1376,8 @@ int yl_tune_chunks(et_plan *p, int mode, int nv, int Dp = 0)
     DevBuf tables;
     for (int round = 0; round < 3; round++) {
         for (size_t ci = 0; ci < cand.size(); ci++) {
-            const int D = cand[ci];
+            const int D = cand[ci].first;
+            yl_use_tiles(p, cand[ci].second);
             YlTable t;
             const std::vector<int> dp{mode == ET_MODE_PROJECT ? D : Dp}, db{D};
             t.project(one, v0, nvv, dp);
@@ -1376,7 +1398,9 @@ int yl_tune_chunks(et_plan *p, int mode, int nv, int Dp = 0)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     p->prof = prof;
-    return cand[std::min_element(best.begin(), best.end()) - best.begin()];
+    const auto &c = cand[std::min_element(best.begin(), best.end()) - best.begin()];
+    yl_use_tiles(p, c.second);
+    return c.first;
 }
 
 // Row-lane caches of an OSEM rebuild: attenuation factors of every view (one
@@ -1400,6 +1424,7 @@ void yl_osem_build(et_plan *p, const float *mu_yf, int S, float eps, const std::
     p->part.ensure((size_t)p->nz * maxsub * yl_plane_elems(p), false);
     p->osem_cur_eps = eps;
     int Dp = p->tune_D, Db = p->tune_D;
+    yl_use_tiles(p, true);  // the tiled projector unless timing prefers the full-width one
     if (p->tune_D <= 0) {
         Dp = yl_tune_chunks(p, ET_MODE_PROJECT, start[1] - start[0]);
         Db = yl_tune_chunks(p, ET_MODE_BPU, start[1] - start[0], Dp);
@@ -3063,6 +3088,13 @@ int et_plan_get_tuning(const et_plan *plan, int *depth_chunks_project, int *dept
     const bool cached = plan->osem_valid;
     *depth_chunks_project = cached ? plan->osem_Dp : 0;
     *depth_chunks_backproject = cached ? plan->osem_Db : 0;
+    return niftyrec_success;
+}
+
+int et_plan_get_projector(const et_plan *plan, int *tiles)
+{
+    if (!plan || !tiles) return niftyrec_error_parameters;
+    *tiles = plan->osem_valid && plan->osem_yl ? plan->yl_tiles : 0;
     return niftyrec_success;
 }
 

@@ -149,10 +149,10 @@ def test_bench_finds_its_traffic_rows():
     for cat in ("project", "bp_rotated_frame", "bp_pull"):
         grid = bench.sweep_grid(cat, 128, 128, 15, 16, tiled=True)
         assert grid == (262144 if cat == "bp_pull" else 15 * 16 * 1024)
-        t = bench.measured_traffic(bench.kernel_of(cat, 5, 128, 128), grid)
+        t = bench.measured_traffic(bench.kernel_of(cat, 5, 128, 128, True), grid)
         assert t is not None and 1e8 < t < 1e9, (cat, t)
-    assert bench.kernel_of("project", 5, 128, 128) == "void (anonymous namespace)::k_yl_project_tile<5, 8>"
-    assert bench.kernel_of("project", 7, 128, 128) == "void (anonymous namespace)::k_yl_project<7, 8, 2>"
+    assert bench.kernel_of("project", 5, 128, 128, True) == "void (anonymous namespace)::k_yl_project_tile<5, 8>"
+    assert bench.kernel_of("project", 5, 128, 128, False) == "void (anonymous namespace)::k_yl_project<5, 8, 2>"
 
 
 def _no_gpu(L):

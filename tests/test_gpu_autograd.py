@@ -148,7 +148,7 @@ def test_plan_osem_fixed_depth_chunks():
     call, the timed choice after it); every chunking gives the oracle's result."""
     act, mu, cams, psf = _case(seed=14, V=8)
     plan = _plan(act.shape, cams, psf)
-    assert plan.tuning() == (0, 0)
+    assert plan.tuning() == (0, 0) and plan.projector() == 0
     y = orc.project(act, mu, cams, psf)
     x, x64 = np.ones_like(act), np.ones(act.shape)
     for s in range(2):
@@ -200,6 +200,9 @@ def test_plan_osem_wide_volumes(shape, V, K):
     y = orc.project(act, mu, cams, psf)
     X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
     plan.osem(X, _t(y), _t(mu), subsets=2, iterations=1)
+    tiles = plan.projector()  # timed choice between the tiled and the full-width projector
+    print("plan osem %s: projector tiles %d (1: full width)" % (shape, tiles))
+    assert tiles == 1 or (K > 1 and shape[2] > 64 and tiles == -(-shape[2] // 64))
     x, x64 = np.ones_like(act), np.ones(act.shape)
     for s in range(2):
         x = orc.mlem_step(x, y[s::2], mu, cams[s::2], psf)

@@ -59,7 +59,8 @@ def test_spect128_osem_iteration_bench_path(spect128):
     X = torch.ones_like(A)
     plan.osem(X, _t(y), M, subsets=8, iterations=1)
     dp, db = plan.tuning()
-    print("spect128 OSEM: depth chunks projection %d, backprojection %d" % (dp, db))
+    print("spect128 OSEM: depth chunks projection %d, backprojection %d; projector tiles %d"
+          % (dp, db, plan.projector()))
     report_parity("spect128 OSEM iteration (8 subsets)", X.cpu().numpy(), _osem_oracle(y, mu, cams, psf, 8),
                   _osem_oracle(y, mu, cams, psf, 8, dtype=np.float64))
 
