@@ -549,6 +549,24 @@ int view_batch(et_plan *p)
     return (int)std::max(1L, std::min<long>(vb, p->V));
 }
 
+// Views per launch of the gradient sweeps.  They run whole rays (one depth
+// chunk: the reverse sum needs the ray's total), so a launch has only
+// slabs x views blocks: at least two per CU are taken, with rotated-frame
+// volumes of up to 4 GiB (256^3: 32 views, 2.2 GB), where view_batch's 256 MiB
+// would give 3 views (48 blocks).
+int grad_view_batch(et_plan *p)
+{
+    if (p->tune_vb > 0) return std::min(p->tune_vb, p->V);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    const long want = (2L * cus + p->slabs - 1) / std::max(p->slabs, 1);
+    const long cap = (long)((4ull << 30) / std::max<size_t>(p->ustride * sizeof(float), 1));
+    const long vb = std::max<long>(view_batch(p), std::min(want, cap));
+    return (int)std::max(1L, std::min<long>(vb, p->V));
+}
+
 void pack(et_plan *p, const float *abi, DevBuf &yf)
 {
     const bool is_mu = &yf == &p->mu_yf;  // attenuation: own guard rows, zero slack after the last column
@@ -892,7 +910,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
     if (p->fast) {
         pack(p, act, p->act_yf);
         pack(p, mu, p->mu_yf);
-        const int vb = view_batch(p);
+        const int vb = grad_view_batch(p);
         p->u.ensure((size_t)vb * p->ustride, true, p->st);
         for (int v0 = 0; v0 < p->V; v0 += vb) {
             const int nv = std::min(vb, p->V - v0);
