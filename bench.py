@@ -3,7 +3,7 @@ PSF 11x11 and attenuation, 8 ordered subsets of 15 views, on one MI355X with
 every input resident in HBM.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-batch]
-                    [--workload spect128|batch48|joint256|hbm_copy] [--depth-chunks D]
+                    [--workload spect128|batch48|joint256|autograd128|hbm_copy] [--depth-chunks D]
 
 A step is one OSEM iteration: 8 subset updates, each = projection of the
 subset's 15 views -> ratio -> reference backprojection -> multiplicative
@@ -284,6 +284,60 @@ def main_joint(args):
     print(json.dumps(line), flush=True)
 
 
+def main_autograd(args):
+    """The differentiable projector at 128^3, 120 views, PSF 11x11, with
+    attenuation: a step is one forward projection of all views through
+    torch.autograd, the Poisson negative log-likelihood of the measured counts,
+    and its backward pass to both the activity (the exact adjoint
+    backprojection) and the attenuation map (the exact attenuation-gradient
+    kernel and its adjoint pull; rotated volumes recomputed per view)."""
+    import torch
+    from niftyrec_amd import api, phantoms
+    from niftyrec_amd.torch_ops import poisson_nll, project
+
+    w = phantoms.workload("spect128")
+    act, mu, psf, cams = w["activity"], w["attenuation"], w["psf"], w["cameras"]
+    dev = torch.device("cuda:0")
+    plan = api.Plan(act.shape, cams, psf)
+    A0 = torch.from_numpy(act).to(dev)
+    M0 = torch.from_numpy(mu).to(dev)
+    Y = torch.from_numpy(phantoms.poisson_data(plan.project(A0, M0).cpu().numpy(), w["counts"], seed=3)).to(dev)
+    A = (A0 * 0.9 + 0.05).contiguous().requires_grad_(True)
+    M = (M0 * 0.8).contiguous().requires_grad_(True)
+
+    def step():
+        A.grad = None
+        M.grad = None
+        loss = poisson_nll(project(A, M, plan), Y)
+        loss.backward()
+        return loss
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    assert torch.isfinite(loss) and torch.isfinite(A.grad).all() and torch.isfinite(M.grad).all()
+    nx, ny, nz = act.shape[2], act.shape[1], act.shape[0]
+    V = len(cams)
+    line = {
+        "metric": "autograd steps/s (projection + Poisson NLL + gradients in activity and mu) at 128^3 x 120 "
+                  "views, PSF 11x11",
+        "value": 1.0 / dt, "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipsoid phantom, Poisson counts %.0e)" % w["counts"],
+        "config": {"workload": "autograd128: %dx%dx%d activity+attenuation, %d views, PSF %dx%d; step = forward "
+                               "projection, Poisson NLL, backward to activity and mu" % (nx, ny, nz, V,
+                                                                                        psf.shape[-1],
+                                                                                        psf.shape[-1])},
+        "gvoxel_views_per_s": V * nx * ny * nz / dt / 1e9,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main_copy(args):
     """Device-memory bandwidth of plain copies (torch clone, read + write) and
     reads (torch row sums) of 64 MiB to 2 GiB: the ceilings the sweep kernels'
@@ -333,6 +387,8 @@ def main():
         return main_batch(args)
     if args.workload == "joint256":
         return main_joint(args)
+    if args.workload == "autograd128":
+        return main_autograd(args)
     if args.workload == "hbm_copy":
         return main_copy(args)
     import torch
