@@ -377,8 +377,10 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 // 2R rows) so that its x' pass needs no other block's columns.  Two LDS planes
 // of (64 + 2R) columns alternate as in the full-width projector; a block then
 // needs under half the LDS and two blocks share a CU, so one block's barriers
-// and gathers overlap the other's.  Each column's arithmetic is the full-width
-// kernel's, in the same order: the results are identical.
+// and gathers overlap the other's.  The x' pass runs first, through LDS, and
+// the axial pass follows in registers on the owned columns only, so the halo
+// columns are blended but not convolved; the separable passes commute, and
+// the results differ from the full-width kernel's only by rounding.
 // ------------------------------------------------------------------------------
 #define YL_TILE ET_YL_TILE
 
