@@ -224,14 +224,14 @@ struct DevBuf {
 // One grouped launch of the row-lane OSEM kernels: argument records and block
 // table at byte offsets of a device table buffer.
 struct YlLaunch {
-    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE
+    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE, YL_GRAD
     int R, RPL, NR, nthr;     // sweep template and block shape (pull: RPL)
     size_t lds;               // dynamic LDS bytes (sweeps)
     size_t args_off, tab_off;
     int n, blocks;
     double bytes;             // algorithmic bytes (the profiling scopes' figure)
 };
-enum { YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4 };
+enum { YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4, YL_GRAD = 5 };
 
 struct et_plan {
     int device = 0;
@@ -272,7 +272,15 @@ struct et_plan {
     int yl_tile_n = 1;                                 // tiles of the tiled projector where it applies, else 1
     size_t yl_lds_full = 0, yl_lds_tile = 0;           // LDS of a full-width / tiled projector block
     DevBuf osem_tables;                                // argument records and block tables of the OSEM launches
+    // row-lane kernels for the single operations of a created plan (projection,
+    // backprojection, exact attenuation gradient with attenuation): factors of
+    // every view in view order, cached on the attenuation map's key
+    bool yl_ops = false;
+    DevBuf fac, sino_t, op_tables;
+    bool fac_valid = false;
+    unsigned long long fac_key = 0;
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
+    std::vector<const void *> osem_sig;                // buffers yl_launches refer to
     DevBuf fp_dev;                  // fingerprint (8 bytes)
     unsigned long long *fp_host = nullptr;  // pinned copy of the fingerprint
     // tuning
@@ -801,6 +809,14 @@ void pull_general(et_plan *p, int v, const float *u, bool adjoint, bool accumula
 // Operations (device pointers, ABI layouts)
 // ---------------------------------------------------------------------------
 
+// Row-lane versions (defined with the row-lane kernels' host code); false when
+// the plan does not take them.
+bool yl_op_project(et_plan *p, const float *act_yf, const float *mu, const float *mu_yf, float *sino, int truncate);
+bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const float *mu_yf, float *out, bool adjoint,
+                       int truncate);
+bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const float *mu, const float *mu_yf,
+                    float *out, int truncate);
+
 void op_project(et_plan *p, const float *act, const float *mu, float *sino, int truncate)
 {
     const float *act_yf = nullptr, *mu_yf = nullptr;
@@ -820,6 +836,7 @@ void op_project(et_plan *p, const float *act, const float *mu, float *sino, int 
                                            (et_stream_t)p->st));
             return;
         }
+        if (yl_op_project(p, act_yf, mu, mu_yf, sino, truncate)) return;
         const int D = mu ? choose_D(p, p->V) : std::max(1, choose_D(p, p->V));
         const float *astart = nullptr;
         if (mu && D > 1) {
@@ -863,6 +880,7 @@ void op_backproject(et_plan *p, const float *sino, const float *mu, float *out, 
             pack(p, mu, p->mu_yf);
             mu_yf = p->mu_yf.p;
         }
+        if (yl_op_backproject(p, sino, mu, mu_yf, out, adjoint, truncate)) return;
         const int vb = view_batch(p);
         p->u.ensure((size_t)vb * p->ustride, true, p->st);
         for (int v0 = 0; v0 < p->V; v0 += vb) {
@@ -910,6 +928,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
     if (p->fast) {
         pack(p, act, p->act_yf);
         pack(p, mu, p->mu_yf);
+        if (exact && yl_op_gradient(p, sino, p->act_yf.p, mu, p->mu_yf.p, out, truncate)) return;
         const int vb = grad_view_batch(p);
         p->u.ensure((size_t)vb * p->ustride, true, p->st);
         for (int v0 = 0; v0 < p->V; v0 += vb) {
@@ -1136,10 +1155,11 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
 // Key of the attenuation map for the OSEM caches: mu_key_launch queues its
 // content fingerprint (copied to pinned host memory) on the plan's stream;
 // mu_key_read returns it once the stream has reached that point (or mu's
-// address when the plan's mu_check is off, 0 without attenuation).
-void mu_key_launch(et_plan *p, const float *mu)
+// address when the plan's mu_check is off and `content` is not asked for, 0
+// without attenuation).
+void mu_key_launch(et_plan *p, const float *mu, bool content = false)
 {
-    if (!mu || !p->mu_check) return;
+    if (!mu || !(p->mu_check || content)) return;
     p->fp_dev.ensure(2, false);
     if (!p->fp_host &&
         hipHostMalloc((void **)&p->fp_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
@@ -1153,10 +1173,10 @@ void mu_key_launch(et_plan *p, const float *mu)
               niftyrec_error_transfergpu);
 }
 
-unsigned long long mu_key_read(const et_plan *p, const float *mu)
+unsigned long long mu_key_read(const et_plan *p, const float *mu, bool content = false)
 {
     if (!mu) return 0;
-    if (!p->mu_check) return (unsigned long long)(uintptr_t)mu;
+    if (!(p->mu_check || content)) return (unsigned long long)(uintptr_t)mu;
     return *p->fp_host;
 }
 
@@ -1212,7 +1232,10 @@ struct YlTable {
                 int gx = 0, gy = 0;
                 args.push_back(fill(i, gx, gy, bytes));
                 bt.add(gx, gy);
-                lds = std::max(lds, kind == YL_BPU ? plans[i]->yl_lds_bpu : plans[i]->yl_lds);
+                const et_plan *q = plans[i];
+                // tiled projector (key NR 0, R > 0), full-width sweeps, backprojection
+                const bool tiled = kind == YL_PROJECT && std::get<2>(g.first) == 0 && q->R > 0;
+                lds = std::max(lds, kind == YL_BPU ? q->yl_lds_bpu : (tiled ? q->yl_lds_tile : q->yl_lds_full));
             }
             const auto &k = g.first;
             L.push_back({kind, std::get<0>(k), std::get<1>(k), std::get<2>(k), std::get<3>(k), lds, tb.put(args),
@@ -1318,10 +1341,13 @@ void yl_issue(const float *dev, const std::vector<YlLaunch> &L, size_t from, siz
         const et_stream_t s = (et_stream_t)st;
         switch (l.kind) {
         case YL_PROJECT:
-        case YL_BPU: {
-            ProfScope ps(prof, l.kind == YL_PROJECT ? ET_PROF_PROJECT : ET_PROF_BPU, l.bytes);
-            check_launch(et_launch_yl_sweep(l.kind == YL_BPU ? 1 : (l.NR == 0 ? 2 : 0), l.R, l.RPL, l.NR, l.nthr, l.lds,
-                                            static_cast<const EtSweepArgs *>(args), tab, l.n, l.blocks, s));
+        case YL_BPU:
+        case YL_GRAD: {
+            ProfScope ps(prof, l.kind == YL_PROJECT ? ET_PROF_PROJECT : (l.kind == YL_BPU ? ET_PROF_BPU : ET_PROF_GRAD),
+                         l.bytes);
+            const int k = l.kind == YL_BPU ? 1 : (l.kind == YL_GRAD ? 3 : (l.NR == 0 ? 2 : 0));
+            check_launch(et_launch_yl_sweep(k, l.R, l.RPL, l.NR, l.nthr, l.lds, static_cast<const EtSweepArgs *>(args),
+                                            tab, l.n, l.blocks, s));
             break;
         }
         case YL_RATIO: {
@@ -1345,6 +1371,164 @@ void yl_upload(DevBuf &buf, const YlTable &t, hipStream_t st)
     check_hip(hipMemcpyAsync(buf.p, t.tb.bytes.data(), t.tb.bytes.size(), hipMemcpyHostToDevice, st),
               niftyrec_error_transfergpu);
     check_hip(hipStreamSynchronize(st), niftyrec_error_transfergpu);
+}
+
+// ---- row-lane single operations -------------------------------------------------
+// A created plan on the row-lane kernels with attenuation projects,
+// backprojects (reference or exact adjoint) and forms the exact attenuation
+// gradient with them: the factors e^{-A} of every view are cached (on the
+// attenuation map's key, as the OSEM caches), the activity is gathered by the
+// projector, and the rotated-frame volumes go to the sweep kernels' pulls
+// (same u layout).
+
+// Factors of every view in view order ([v][z'][x'][rows]) for the map mu (mu_yf
+// its YF copy), rebuilt when the map's content fingerprint changes (whatever
+// the plan's mu_check, which keys the OSEM caches: these calls may see other
+// maps between OSEM calls, and a map rewritten in place is caught).
+const float *yl_view_factors(et_plan *p, const float *mu, const float *mu_yf)
+{
+    mu_key_launch(p, mu, true);
+    check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
+    const unsigned long long key = mu_key_read(p, mu, true);
+    if (p->fac_valid && p->fac_key == key) return p->fac.p;
+    p->fac_valid = false;
+    p->fac.ensure(yl_att_elems(p, p->V), false);
+    EtSweepArgs a = yl_sweep_args(p, p->d_recs.p, p->V, 1, p->fac.p);
+    a.mu = mu_yf;
+    {
+        ProfScope ps(p, ET_PROF_OTHER, (double)p->V * 8.0 * p->nvox);
+        check_launch(et_launch_yl_factors(p->yl_rpl, a, (et_stream_t)p->st));
+    }
+    p->fac_valid = true;
+    p->fac_key = key;
+    return p->fac.p;
+}
+
+// Depth chunks of a row-lane launch of `per_chunk` blocks per chunk: the fewest
+// (powers of two, chunks of at least 4 planes) that give two blocks per CU.
+int yl_op_chunks(const et_plan *p, long per_chunk)
+{
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        cus = 256;
+    int D = 1;
+    while (D < 32 && p->nz / (2 * D) >= 4 && per_chunk * D < 2L * cus) D *= 2;
+    return D;
+}
+
+// The plan's sinogram [V][ny][nx] in the row-lane layout [V][x'][rows].
+float *yl_sino_t(et_plan *p, const float *sino)
+{
+    float *t = p->sino_t.ensure((size_t)p->V * yl_plane_elems(p), false);
+    ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->V * p->P);
+    check_launch(et_launch_gather_planes_t(t, sino, nullptr, p->V, p->nx, p->ny, p->yu, (et_stream_t)p->st));
+    return t;
+}
+
+void yl_run(et_plan *p, const YlTable &t)
+{
+    yl_upload(p->op_tables, t, p->st);
+    yl_issue(p->op_tables.p, t.L, 0, t.L.size(), p->st, p);
+}
+
+bool yl_op_project(et_plan *p, const float *act_yf, const float *mu, const float *mu_yf, float *sino, int truncate)
+{
+    if (!p->yl_ops || !p->yl || !act_yf || !mu) return false;
+    const float *fac = yl_view_factors(p, mu, mu_yf);
+    const int D = yl_op_chunks(p, (long)p->V * p->yl_tiles);
+    int Deff, zlen;
+    yl_chunks(p, D, Deff, zlen);
+    float *part = p->part.ensure((size_t)Deff * p->V * yl_plane_elems(p), false);
+    YlTable t;
+    t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::project_key, [&](int, int &gx, int &gy, double &bytes) {
+        bytes += (double)p->V * (8.0 * p->nvox + 4.0 * p->P);
+        EtSweepArgs a = yl_sweep_args(p, p->d_recs.p, p->V, D, fac);
+        a.act = const_cast<float *>(act_yf);
+        a.part = part;
+        gx = a.D * p->yl_tiles;
+        gy = p->V;
+        return a;
+    });
+    yl_run(p, t);
+    ProfScope ps(p, ET_PROF_OTHER, (Deff + 1.0) * 4.0 * p->V * p->P);
+    check_launch(
+        et_launch_yl_parts_to_sino(part, Deff, p->V, p->nx, p->ny, p->yu, truncate, sino, (et_stream_t)p->st));
+    return true;
+}
+
+bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const float *mu_yf, float *out, bool adjoint,
+                       int truncate)
+{
+    if (!p->yl_ops || !p->yl || !mu) return false;
+    const float *fac = yl_view_factors(p, mu, mu_yf);
+    const size_t pe = yl_plane_elems(p);
+    const float *st = yl_sino_t(p, sino);
+    const int vb = view_batch(p);
+    p->u.ensure((size_t)vb * p->ustride, true, p->st);
+    for (int v0 = 0; v0 < p->V; v0 += vb) {
+        const int nv = std::min(vb, p->V - v0);
+        const int D = yl_op_chunks(p, nv);
+        const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
+        YlTable t;
+        t.grouped<EtSweepArgs>(YL_BPU, {p}, YlTable::sweep_key, [&](int, int &gx, int &gy, double &bytes) {
+            bytes += (double)nv * (8.0 * p->nvox + 4.0 * p->P);
+            EtSweepArgs a = yl_sweep_args(p, recs, nv, D, fac + (size_t)v0 * p->nz * pe);
+            a.src = ET_SRC_SINO;
+            a.sino = st + (size_t)v0 * pe;
+            a.u = p->u.p;
+            if (p->K > 0 && adjoint) a.psf = p->d_sep_flip.p;  // correlation: the adjoint of the PSF
+            gx = a.D;
+            gy = nv;
+            return a;
+        });
+        yl_run(p, t);
+        const bool last = v0 + nv >= p->V;
+        pull_fast(p, recs, nv, p->u.p, adjoint, ET_EPI_ABI_STORE, v0 > 0, last && truncate, out);
+    }
+    return true;
+}
+
+// Exact attenuation gradient: per view batch, the projection's partial sums
+// (full-width projector, the gradient sweep's chunking), the gradient sweep
+// and the exact adjoint pull.
+bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const float *mu, const float *mu_yf,
+                    float *out, int truncate)
+{
+    if (!p->yl_ops || !p->yl) return false;
+    const float *fac = yl_view_factors(p, mu, mu_yf);
+    const size_t pe = yl_plane_elems(p);
+    const float *st = yl_sino_t(p, sino);
+    const int vb = view_batch(p);
+    p->u.ensure((size_t)vb * p->ustride, true, p->st);
+    for (int v0 = 0; v0 < p->V; v0 += vb) {
+        const int nv = std::min(vb, p->V - v0);
+        const int D = yl_op_chunks(p, nv);
+        int Deff, zlen;
+        yl_chunks(p, D, Deff, zlen);
+        float *part = p->part.ensure((size_t)Deff * nv * pe, false);
+        const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
+        auto args = [&](int &gx, int &gy, double &bytes) {
+            bytes += (double)nv * (8.0 * p->nvox + 4.0 * p->P);
+            EtSweepArgs a = yl_sweep_args(p, recs, nv, D, fac + (size_t)v0 * p->nz * pe);
+            a.act = const_cast<float *>(act_yf);
+            a.part = part;
+            a.sino = st + (size_t)v0 * pe;
+            a.u = p->u.p;
+            gx = a.D;
+            gy = nv;
+            return a;
+        };
+        YlTable t;
+        t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::sweep_key,
+                               [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
+        t.grouped<EtSweepArgs>(YL_GRAD, {p}, YlTable::sweep_key,
+                               [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
+        yl_run(p, t);
+        const bool last = v0 + nv >= p->V;
+        pull_fast(p, recs, nv, p->u.p, true, ET_EPI_ABI_STORE, v0 > 0, last && truncate, out);
+    }
+    return true;
 }
 
 // Projector kind of a row-lane plan: the tiled kernel (where it applies) or the
@@ -1421,6 +1605,33 @@ int yl_tune_chunks(et_plan *p, int mode, int nv, int Dp = 0)
     return c.first;
 }
 
+// Device buffers the row-lane OSEM launch table refers to.  A single
+// operation of the plan may grow (reallocate) some of them between OSEM calls;
+// the table is then rebuilt.
+std::vector<const void *> yl_osem_sig(const et_plan *p)
+{
+    return {p->x_yf.p, p->osem_meas.p, p->part.p, p->ratio.p, p->u.p, p->sens.p, p->osem_recs.p, p->osem_E.p};
+}
+
+// Launch table of one row-lane OSEM iteration: per subset projection, ratio,
+// backprojection, pull with the update.
+void yl_osem_tables(et_plan *p, int S, const std::vector<int> &start)
+{
+    const std::vector<et_plan *> one{p};
+    YlTable t;
+    for (int s = 0; s < S; s++) {
+        const std::vector<int> v0{start[s]}, nv{start[s + 1] - start[s]}, dp{p->osem_Dp}, db{p->osem_Db};
+        const std::vector<const float *> sens{p->sens.p + (size_t)s * p->yf_elems};
+        t.project(one, v0, nv, dp);
+        t.ratio(one, v0, nv, dp);
+        t.bpu(one, v0, nv, db, false);
+        t.pull(one, v0, nv, sens, nullptr);
+    }
+    yl_upload(p->osem_tables, t, p->st);
+    p->yl_launches = t.L;
+    p->osem_sig = yl_osem_sig(p);
+}
+
 // Row-lane caches of an OSEM rebuild: attenuation factors of every view (one
 // launch), chunkings, subset sensitivities, and the launch table of one
 // iteration.  The measured planes and the iterate are set by the caller.
@@ -1464,18 +1675,7 @@ void yl_osem_build(et_plan *p, const float *mu_yf, int S, float eps, const std::
         ProfScope ps(p, ET_PROF_OTHER, 0);
         check_launch(et_launch_yf_clamp_min(out[0], p->nx, p->ny, p->nz, p->yp, p->gy, eps, (et_stream_t)p->st));
     }
-    // launch table of one iteration: per subset projection, ratio, backprojection, pull
-    YlTable t;
-    for (int s = 0; s < S; s++) {
-        const std::vector<int> v0{start[s]}, nv{start[s + 1] - start[s]}, dp{p->osem_Dp}, db{p->osem_Db};
-        const std::vector<const float *> sens{p->sens.p + (size_t)s * p->yf_elems};
-        t.project(one, v0, nv, dp);
-        t.ratio(one, v0, nv, dp);
-        t.bpu(one, v0, nv, db, false);
-        t.pull(one, v0, nv, sens, nullptr);
-    }
-    yl_upload(p->osem_tables, t, p->st);
-    p->yl_launches = t.L;
+    yl_osem_tables(p, S, start);
 }
 
 void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int S, float eps,
@@ -1536,6 +1736,8 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
                                                    p->V, p->nx, p->ny, p->yu, (et_stream_t)p->st));
         }
         pack(p, x, p->x_yf);
+        // a single operation reallocated a buffer of the cached table
+        if (yl_osem_sig(p) != p->osem_sig) yl_osem_tables(p, S, start);
         p->osem_start = start;
         p->osem_cur_mu = mu_yf;
         p->osem_cur_eps = eps;
@@ -3018,6 +3220,7 @@ int et_plan_create(et_plan **plan, int nx, int ny, int nz, const float *cameras,
             delete p;
             return rc;
         }
+        p->yl_ops = p->yl;
         *plan = p;
         return niftyrec_success;
     });
@@ -3089,6 +3292,7 @@ int et_plan_invalidate(et_plan *plan)
 {
     if (!plan) return niftyrec_error_parameters;
     plan->osem_valid = false;
+    plan->fac_valid = false;
     return niftyrec_success;
 }
 

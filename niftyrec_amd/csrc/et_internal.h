@@ -175,8 +175,9 @@ int et_launch_partial_line(const float *act, const float *mu, const float *bgimg
 int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_stream_t stream);
 // Row-lane kernels of the OSEM subset loop (et_yl.h).  Grouped launches only
 // (tab as et_launch_sweep_group): sweeps kind 0 projection, 1 rotated-frame
-// backprojection, gx = depth chunks, gy = views, nthr = 16 * W threads, `lds`
-// bytes of LDS; kind 2 the tiled projection (R > 0), gx = depth chunks x
+// backprojection, 3 exact attenuation gradient (reads the projection's partial
+// sums of the same chunking), gx = depth chunks, gy = views, nthr = 16 * W
+// threads, `lds` bytes of LDS; kind 2 the tiled projection (R > 0), gx = depth chunks x
 // tiles (tile fastest), 512 threads, NR unused; pull gx = ceil(nx * nz / 16),
 // gy = 1, 256 threads.  RPL (rows per lane) in {2, 4, 6, 8}, NR (column
 // rounds) in {1, 2, 3}.
@@ -193,7 +194,11 @@ int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void 
                       et_stream_t stream);
 // Attenuation factors of a.V views into a.att (row-lane layout) from a.mu (YF).
 int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream);
-// dst[v][x][y] = src[order[v]][y][x] (y < ny), 0 for ny <= y < rows.
+// sino[v][y][x] = sum over d < D of part[d][v][x][y] (compensated), truncated
+// at 0 if asked: row-lane partial sums (V views, rows per column) in the ABI layout.
+int et_launch_yl_parts_to_sino(const float *part, int D, int V, int nx, int ny, int rows, int truncate, float *sino,
+                               et_stream_t stream);
+// dst[v][x][y] = src[order[v]][y][x] (y < ny), 0 for ny <= y < rows (order null: v).
 int et_launch_gather_planes_t(float *dst, const float *src, const int *order, int V, int nx, int ny, int rows,
                               et_stream_t stream);
 // Axial rows per block of a sweep launch (mode, PSF radius).

@@ -648,7 +648,8 @@ __global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict_
 
 // dst[v][x][y] = src[order[v]][y][x] for y < ny, 0 for ny <= y < rows: the
 // measured planes of the row-lane kernels, gathered in subset order and
-// transposed (through a 32 x 33 LDS tile).  grid (ceil(nx/32), ceil(rows/32), V).
+// transposed (through a 32 x 33 LDS tile); order null: view order.  grid
+// (ceil(nx/32), ceil(rows/32), V).
 __global__ void __launch_bounds__(256) k_gather_planes_t(float *__restrict__ dst, const float *__restrict__ src,
                                                          const int *__restrict__ order, int nx, int ny, int rows)
 {
@@ -656,7 +657,7 @@ __global__ void __launch_bounds__(256) k_gather_planes_t(float *__restrict__ dst
     const int v = blockIdx.z;
     const int xb = blockIdx.x * 32, yb = blockIdx.y * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    const float *sp = src + (size_t)order[v] * nx * ny;
+    const float *sp = src + (size_t)(order ? order[v] : v) * nx * ny;
     for (int k = ty; k < 32; k += 8) {
         const int x = xb + tx, y = yb + k;
         tile[k][tx] = (x < nx && y < ny) ? sp[x + (size_t)nx * y] : 0.f;
@@ -666,6 +667,34 @@ __global__ void __launch_bounds__(256) k_gather_planes_t(float *__restrict__ dst
     for (int k = ty; k < 32; k += 8) {
         const int y = yb + tx, x = xb + k;
         if (x < nx && y < rows) dp[y + (size_t)rows * x] = tile[tx][k];
+    }
+}
+
+// sino[v][y][x] = sum over d < D of part[d][v][x][y] (row-lane partial sums,
+// rows per column), added in order of d with TwoSum compensation, truncated at
+// 0 if asked: the projection of the row-lane projector in the ABI layout,
+// transposed through a 32 x 33 LDS tile.  grid (ceil(nx/32), ceil(ny/32), V).
+__global__ void __launch_bounds__(256) k_yl_parts_to_sino(const float *__restrict__ part, int D, int V, int nx,
+                                                          int ny, int rows, int truncate, float *__restrict__ sino)
+{
+    __shared__ float tile[32][33];
+    const int v = blockIdx.z;
+    const int xb = blockIdx.x * 32, yb = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const size_t stride = (size_t)V * nx * rows;
+    for (int k = ty; k < 32; k += 8) {
+        const int x = xb + k, y = yb + tx;
+        tile[k][tx] = (x < nx && y < ny) ? sum_parts(part, D, stride, ((size_t)v * nx + x) * rows + y) : 0.f;
+    }
+    __syncthreads();
+    float *sp = sino + (size_t)v * nx * ny;
+    for (int k = ty; k < 32; k += 8) {
+        const int y = yb + k, x = xb + tx;
+        if (x < nx && y < ny) {
+            float val = tile[tx][k];
+            if (truncate && val < 0.f) val = 0.f;
+            sp[x + (size_t)nx * y] = val;
+        }
     }
 }
 
@@ -1003,6 +1032,16 @@ int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream)
     ET_YLF(2) ET_YLF(4) ET_YLF(6) ET_YLF(8)
 #undef ET_YLF
     return (int)hipErrorInvalidValue;
+}
+
+int et_launch_yl_parts_to_sino(const float *part, int D, int V, int nx, int ny, int rows, int truncate, float *sino,
+                               et_stream_t stream)
+{
+    if (D <= 0 || V <= 0 || nx <= 0 || ny <= 0 || rows < ny) return (int)hipErrorInvalidValue;
+    dim3 g((nx + 31) / 32, (ny + 31) / 32, V);
+    hipLaunchKernelGGL(k_yl_parts_to_sino, g, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), part, D, V, nx, ny,
+                       rows, truncate, sino);
+    return (int)hipGetLastError();
 }
 
 int et_launch_gather_planes_t(float *dst, const float *src, const int *order, int V, int nx, int ny, int rows,

@@ -570,7 +570,7 @@ static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_
 }
 
 // Row-lane sweeps (et_yl.h) of PSF radius R: kind 0 projection, 1 rotated-frame
-// backprojection; a grouped launch of `blocks` blocks of nthr = 16 * W threads
+// backprojection, 2 tiled projection, 3 exact attenuation gradient; a grouped launch of `blocks` blocks of nthr = 16 * W threads
 // with `lds` bytes of dynamic LDS (two planes of the widest problem).
 template <int R>
 static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args, const int4 *tab,
@@ -594,6 +594,9 @@ static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const Et
         if (kind == 0) {                                                                                          \
             allow_lds(k_yl_project<R, rpl, nr>, lds);                                                             \
             hipLaunchKernelGGL((k_yl_project<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);      \
+        } else if (kind == 3) {                                                                                   \
+            allow_lds(k_yl_grad<R, rpl, nr>, lds);                                                                \
+            hipLaunchKernelGGL((k_yl_grad<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);         \
         } else {                                                                                                  \
             allow_lds(k_yl_bpu<R, rpl, nr>, lds);                                                                 \
             hipLaunchKernelGGL((k_yl_bpu<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);          \

@@ -371,6 +371,141 @@ __device__ __forceinline__ void yl_project_body(const EtSweepArgs &a, const Blk 
 }
 
 // ------------------------------------------------------------------------------
+// Exact attenuation gradient in the rotated frame, with cached factors:
+//   u(x', y, z') = -s(x', y) ( sum_{k >= z'} PSF_k(act')(x', y, k) e^{-A(k)}
+//                              + bg e^{-A_tot} )
+// (the derivative of the projection with respect to mu'(z'); the sweep kernels'
+// k_grad_mu forms the same sum from the attenuation suffix).  Block (d, v)
+// sweeps its chunk's planes from the far end toward the detector; the sum of
+// the later chunks comes from the projector's partial sums of the same
+// chunking, part[d'][v][x'][rows] (the last chunk's holds the bg term), added
+// with TwoSum in order of d', and the running sum is Kahan-compensated.  s is
+// in the row-lane layout [v][x'][rows]; u as the backprojection's.
+// ------------------------------------------------------------------------------
+template <int R, int RPL, int NR>
+__device__ __forceinline__ void yl_grad_body(const EtSweepArgs &a, const Blk &b)
+{
+    constexpr int K = 2 * R + 1;
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4, W = blockDim.x >> 4;
+    const int d = b.x, v = b.y;
+    const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
+    constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
+    const int row0 = RPL * lane;
+    const int dx = a.yp, dz = a.yp * a.xp;
+    const int PF = yl_plane_floats(a.nx, R, rows);
+    const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
+    if constexpr (R > 0) yl_zero_lds(lds, a.yl_nbuf * PF);
+    const float *act = a.act + a.gy + row0;
+    float *uv = a.u + (size_t)v * a.ustride + row0;
+
+    float acc[NR][RPL], cmp[NR][RPL];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int xc = r * W + slot;
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            acc[r][i] = 0.f;
+            cmp[r][i] = 0.f;
+        }
+        if (xc >= a.nx) continue;
+        float hi[RPL], lo[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            hi[i] = 0.f;
+            lo[i] = 0.f;
+        }
+        for (int d2 = d + 1; d2 < a.D; d2++) {
+            float t[RPL];
+            yl_load<RPL>(a.part + (((size_t)d2 * a.V + v) * a.nx + xc) * rows + row0, t);
+#pragma unroll
+            for (int i = 0; i < RPL; i++) comp_add(hi[i], lo[i], t[i]);
+        }
+        if (d == a.D - 1 && a.bg_line != 0.f) {
+            float e[RPL];
+            yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + a.nz - 1) * a.nx + xc) * rows + row0, e);
+#pragma unroll
+            for (int i = 0; i < RPL; i++) comp_add(hi[i], lo[i], a.bg_line * e[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            acc[r][i] = hi[i];
+            cmp[r][i] = -lo[i];  // the running sum is acc - cmp
+        }
+    }
+    auto kahan = [&](int r, int i, float h, float e) {
+        const float y = fmaf(h, e, -cmp[r][i]);
+        const float t = acc[r][i] + y;
+        cmp[r][i] = (t - acc[r][i]) - y;
+        acc[r][i] = t;
+    };
+    // s is read again at every plane (a cache hit after the first) rather than
+    // held in registers across the sweep
+    auto store = [&](int r, int xc, int z) {
+        float sv[RPL], out[RPL];
+        yl_load<RPL>(a.sino + ((size_t)v * a.nx + xc) * rows + row0, sv);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) out[i] = -sv[i] * (acc[r][i] - cmp[r][i]);
+        yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), out);
+    };
+
+    constexpr int PB = 16 / NR;  // planes per sample batch (descending)
+    YlSamp sb;
+    int kb = 0;
+    for (int z = ze - 1; z >= zb; z--, kb = kb + 1 == PB ? 0 : kb + 1) {
+        const float *kz = psf_plane<R>(a.psf, z);  // never read when R == 0
+        if (kb == 0)  // (planes before the chunk and columns past the end give samples no round takes)
+            sb = yl_samp_of(fast_sample(fv, (float)((lane % NR) * W + slot), z - lane / NR, a.nx, a.nz, dx, dz));
+        float e[NR][RPL];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = r * W + slot;
+            if (xc < a.nx) yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+        }
+        if constexpr (R > 0) {
+            float *plane = lds + (a.yl_nbuf == 2 ? (z & 1) * PF : 0);
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) {
+                    const Samp s = yl_samp_from(sb, kb * NR + r);
+                    float bl[RPL], cp[RPL];
+                    yl_bilinear<RPL>(act + s.off, dx, dz, s, bl);
+                    yl_column_pass<R, RPL>(bl, kz + K, cp);
+                    lds_store<RPL>(plane + (xc + R) * CS + row0, cp);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) {
+                    float h[RPL];
+                    yl_row_pass<R, RPL>(plane, CS, xc, row0, kz, h);
+#pragma unroll
+                    for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
+                    store(r, xc, z);
+                }
+            }
+            if (a.yl_nbuf == 1) __syncthreads();  // one plane: the next writes wait for these reads
+        } else {
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const int xc = r * W + slot;
+                if (xc < a.nx) {
+                    const Samp s = yl_samp_from(sb, kb * NR + r);
+                    float h[RPL];
+                    yl_bilinear<RPL>(act + s.off, dx, dz, s, h);
+#pragma unroll
+                    for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
+                    store(r, xc, z);
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
 // Tiled projection: as yl_project_body, but block (tile, d, v) of 512 threads
 // (32 lane rows) owns YL_TILE = 64 columns of each plane, two per row, and
 // also computes the R columns on either side of its tile (a third round of
@@ -575,6 +710,17 @@ __global__ void __launch_bounds__(512, 4) k_yl_project_tile(const EtSweepArgs *_
     if (!group_block(tab, n, b, j)) return;
     const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
     yl_project_tile_body<R, RPL>(a, b);
+}
+
+template <int R, int RPL, int NR>
+__global__ void __launch_bounds__(1024) k_yl_grad(const EtSweepArgs *__restrict__ args, const int4 *__restrict__ tab,
+                                                  int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
+    yl_grad_body<R, RPL, NR>(a, b);
 }
 
 // One LDS plane per block; where the kernel needs at most 64 VGPRs (8 waves
