@@ -224,14 +224,14 @@ struct DevBuf {
 // One grouped launch of the row-lane OSEM kernels: argument records and block
 // table at byte offsets of a device table buffer.
 struct YlLaunch {
-    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE, YL_GRAD
+    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE, YL_GRAD, YL_PULL_ADJ
     int R, RPL, NR, nthr;     // sweep template and block shape (pull: RPL)
     size_t lds;               // dynamic LDS bytes (sweeps)
     size_t args_off, tab_off;
     int n, blocks;
     double bytes;             // algorithmic bytes (the profiling scopes' figure)
 };
-enum { YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4, YL_GRAD = 5 };
+enum { YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4, YL_GRAD = 5, YL_PULL_ADJ = 6 };
 
 struct et_plan {
     int device = 0;
@@ -276,7 +276,7 @@ struct et_plan {
     // backprojection, exact attenuation gradient with attenuation): factors of
     // every view in view order, cached on the attenuation map's key
     bool yl_ops = false;
-    DevBuf fac, sino_t, op_tables;
+    DevBuf fac, sino_t, op_tables, op_acc;  // (op_acc: backprojection accumulator, YF layout)
     bool fac_valid = false;
     unsigned long long fac_key = 0;
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
@@ -1355,6 +1355,11 @@ void yl_issue(const float *dev, const std::vector<YlLaunch> &L, size_t from, siz
             check_launch(et_launch_ratio_group(static_cast<const EtRatioArgs *>(args), tab, l.n, l.blocks, s));
             break;
         }
+        case YL_PULL_ADJ: {
+            ProfScope ps(prof, ET_PROF_PULL, l.bytes);
+            check_launch(et_launch_yl_pull_adj(l.RPL, static_cast<const EtPullArgs *>(args), tab, l.n, l.blocks, s));
+            break;
+        }
         default: {
             ProfScope ps(prof, ET_PROF_PULL, l.bytes);
             check_launch(et_launch_yl_pull(l.kind == YL_PULL_OSEM ? ET_EPI_OSEM : ET_EPI_YF_STORE, l.RPL,
@@ -1432,6 +1437,22 @@ void yl_run(et_plan *p, const YlTable &t)
     yl_issue(p->op_tables.p, t.L, 0, t.L.size(), p->st, p);
 }
 
+// Adds to t the pull of nv views' rotated-frame volumes (p->u) into the YF
+// accumulator acc: the reference pull or the exact adjoint; accumulate onto acc
+// after the first batch, truncate at the last.
+void yl_pull_into(YlTable &t, et_plan *p, const float *recs, int nv, bool adjoint, bool accumulate, bool truncate,
+                  float *acc)
+{
+    t.grouped<EtPullArgs>(adjoint ? YL_PULL_ADJ : YL_PULL_STORE, {p}, YlTable::pull_key,
+                          [&](int, int &gx, int &gy, double &bytes) {
+                              bytes += (double)nv * 4.0 * p->nvox + (accumulate ? 8.0 : 4.0) * p->nvox;
+                              gx = (int)(((size_t)p->nx * p->nz + 15) / 16);
+                              gy = 1;
+                              return pull_args(p, recs, nv, p->u.p, adjoint, ET_EPI_YF_STORE, accumulate, truncate,
+                                               acc, nullptr, nullptr);
+                          });
+}
+
 bool yl_op_project(et_plan *p, const float *act_yf, const float *mu, const float *mu_yf, float *sino, int truncate)
 {
     if (!p->yl_ops || !p->yl || !act_yf || !mu) return false;
@@ -1466,11 +1487,12 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
     const float *st = yl_sino_t(p, sino);
     const int vb = view_batch(p);
     p->u.ensure((size_t)vb * p->ustride, true, p->st);
+    float *acc = p->op_acc.ensure(p->yf_elems, true, p->st);
+    YlTable t;  // every batch in one table: the batches reuse u in stream order
     for (int v0 = 0; v0 < p->V; v0 += vb) {
         const int nv = std::min(vb, p->V - v0);
         const int D = yl_op_chunks(p, nv);
         const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
-        YlTable t;
         t.grouped<EtSweepArgs>(YL_BPU, {p}, YlTable::sweep_key, [&](int, int &gx, int &gy, double &bytes) {
             bytes += (double)nv * (8.0 * p->nvox + 4.0 * p->P);
             EtSweepArgs a = yl_sweep_args(p, recs, nv, D, fac + (size_t)v0 * p->nz * pe);
@@ -1482,10 +1504,11 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
             gy = nv;
             return a;
         });
-        yl_run(p, t);
-        const bool last = v0 + nv >= p->V;
-        pull_fast(p, recs, nv, p->u.p, adjoint, ET_EPI_ABI_STORE, v0 > 0, last && truncate, out);
+        yl_pull_into(t, p, recs, nv, adjoint, v0 > 0, v0 + nv >= p->V && truncate, acc);
     }
+    yl_run(p, t);
+    ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
+    check_launch(et_launch_unpack_yf(acc, out, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
     return true;
 }
 
@@ -1501,12 +1524,20 @@ bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const fl
     const float *st = yl_sino_t(p, sino);
     const int vb = view_batch(p);
     p->u.ensure((size_t)vb * p->ustride, true, p->st);
+    float *acc = p->op_acc.ensure(p->yf_elems, true, p->st);
+    // every batch in one table (the batches reuse part and u in stream order);
+    // the partial sums take the most chunks any batch uses
+    int Dmax = 1;
+    for (int v0 = 0; v0 < p->V; v0 += vb) {
+        int Deff, zlen;
+        yl_chunks(p, yl_op_chunks(p, std::min(vb, p->V - v0)), Deff, zlen);
+        Dmax = std::max(Dmax, Deff);
+    }
+    float *part = p->part.ensure((size_t)Dmax * vb * pe, false);
+    YlTable t;
     for (int v0 = 0; v0 < p->V; v0 += vb) {
         const int nv = std::min(vb, p->V - v0);
         const int D = yl_op_chunks(p, nv);
-        int Deff, zlen;
-        yl_chunks(p, D, Deff, zlen);
-        float *part = p->part.ensure((size_t)Deff * nv * pe, false);
         const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
         auto args = [&](int &gx, int &gy, double &bytes) {
             bytes += (double)nv * (8.0 * p->nvox + 4.0 * p->P);
@@ -1519,15 +1550,15 @@ bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const fl
             gy = nv;
             return a;
         };
-        YlTable t;
         t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::sweep_key,
                                [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
         t.grouped<EtSweepArgs>(YL_GRAD, {p}, YlTable::sweep_key,
                                [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
-        yl_run(p, t);
-        const bool last = v0 + nv >= p->V;
-        pull_fast(p, recs, nv, p->u.p, true, ET_EPI_ABI_STORE, v0 > 0, last && truncate, out);
+        yl_pull_into(t, p, recs, nv, true, v0 > 0, v0 + nv >= p->V && truncate, acc);
     }
+    yl_run(p, t);
+    ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
+    check_launch(et_launch_unpack_yf(acc, out, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
     return true;
 }
 

@@ -192,6 +192,9 @@ int et_launch_yl_sweep(int kind, int R, int RPL, int NR, int nthr, size_t lds, c
                        const void *tab, int n, int blocks, et_stream_t stream);
 int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void *tab, int n, int blocks,
                       et_stream_t stream);
+// Exact adjoint pull of the row-lane rotated-frame volumes into a.out (YF layout,
+// accumulate / truncate as ET_EPI_YF_STORE); grid as the row-lane pull.
+int et_launch_yl_pull_adj(int RPL, const EtPullArgs *args, const void *tab, int n, int blocks, et_stream_t stream);
 // Attenuation factors of a.V views into a.att (row-lane layout) from a.mu (YF).
 int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream);
 // sino[v][y][x] = sum over d < D of part[d][v][x][y] (compensated), truncated

@@ -1020,6 +1020,21 @@ int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void 
     return (int)hipErrorInvalidValue;
 }
 
+int et_launch_yl_pull_adj(int RPL, const EtPullArgs *args, const void *tab, int n, int blocks, et_stream_t stream)
+{
+    if (n <= 0 || blocks <= 0) return (int)hipErrorInvalidValue;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int4 *t = reinterpret_cast<const int4 *>(tab);
+#define ET_YLA(rpl)                                                                                  \
+    if (RPL == rpl) {                                                                                \
+        hipLaunchKernelGGL((k_yl_pull_adj<rpl>), dim3(blocks), dim3(256), 0, st, args, t, n);        \
+        return (int)hipGetLastError();                                                               \
+    }
+    ET_YLA(2) ET_YLA(4) ET_YLA(6) ET_YLA(8)
+#undef ET_YLA
+    return (int)hipErrorInvalidValue;
+}
+
 int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream)
 {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);

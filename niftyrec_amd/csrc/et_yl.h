@@ -845,6 +845,97 @@ __device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
     }
 }
 
+// ------------------------------------------------------------------------------
+// Exact adjoint of the row-lane and sweep projectors' gather: image column
+// (x, z) takes, from every rotated-frame column (x', z') whose forward sample
+// has (x, z) among its corners, u(x', z') times the forward weight of that
+// corner, summed over views in order and candidates x'-major (the order of the
+// sweep kernels' adjoint pull, so each element sees the same FMA sequence).
+// The candidates lie in a box of at most 3 x 3 columns (bounded through the
+// inverse map); lane k of the row computes candidate k's weight and offset,
+// and the row takes them in turn by cross-lane permutes.  Stores as
+// ET_EPI_YF_STORE (accumulate, truncate; rows below ny).  Block: 16 image
+// columns, column blocks contiguous per XCD.  256 threads.
+// ------------------------------------------------------------------------------
+template <int RPL>
+__device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk &b)
+{
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int cb = ((b.gx & 7) == 0) ? (b.lin & 7) * (b.gx >> 3) + (b.lin >> 3) : b.lin;
+    const int c = cb * 16 + slot;
+    if (c >= a.nx * a.nz) return;  // (the whole 16-lane row)
+    const int x = c % a.nx, z = c / a.nx;
+    constexpr int rows = 16 * RPL;  // == a.yu
+    const int row0 = RPL * lane;
+    const float fxo = (float)x, fzo = (float)z;
+    const int pa = ((int)threadIdx.x & ~15) * 4;  // permute address of lane 0 of this row
+    float acc[RPL];
+#pragma unroll
+    for (int i = 0; i < RPL; i++) acc[i] = 0.f;
+    for (int v = 0; v < a.V; v++) {
+        const float *rec = a.views + (size_t)v * ET_VIEW_REC;
+        const float *uv = a.u + (size_t)v * a.ustride + row0;
+        const float f00 = cload(rec + 0), f02 = cload(rec + 2), f03 = cload(rec + 3), f20 = cload(rec + 8),
+                    f22 = cload(rec + 10), f23 = cload(rec + 11);
+        const float i00 = cload(rec + 24), i02 = cload(rec + 26), i20 = cload(rec + 30), i22 = cload(rec + 32);
+        const float rx = fxo - f03, rz = fzo - f23;
+        const float cx = i00 * rx + i02 * rz, cz = i20 * rx + i22 * rz;
+        const float hx = fabsf(i00) + fabsf(i02) + 0.01f, hz = fabsf(i20) + fabsf(i22) + 0.01f;
+        const int xlo = max(0, (int)ceilf(cx - hx)), xhi = min(a.nx - 1, (int)floorf(cx + hx));
+        const int zlo = max(0, (int)ceilf(cz - hz)), zhi = min(a.nz - 1, (int)floorf(cz + hz));
+        const int bz = zhi - zlo + 1;
+        const int nc = (xhi >= xlo && bz > 0) ? (xhi - xlo + 1) * bz : 0;
+        for (int k0 = 0; k0 < nc; k0 += 16) {
+            float wk = 0.f;
+            int ok = 0;
+            const int kk = k0 + lane;
+            if (kk < nc) {
+                const int xc = xlo + kk / bz, zc = zlo + kk % bz;
+                const float px = pos2(f00, (float)xc, f02, (float)zc, f03);
+                const float pz = pos2(f20, (float)xc, f22, (float)zc, f23);
+                const float fx = floorf(px), fz = floorf(pz);
+                const float rxx = px - fx, rzz = pz - fz;
+                const float wx = (fx == fxo) ? 1.f - rxx : ((fx == fxo - 1.f) ? rxx : 0.f);
+                const float wz = (fz == fzo) ? 1.f - rzz : ((fz == fzo - 1.f) ? rzz : 0.f);
+                wk = wx * wz;
+                ok = rows * ((xc + 1) + a.xp * (zc + 1));
+            }
+            const int m = min(16, nc - k0);
+            for (int k = 0; k < m; k++) {
+                const float w = __int_as_float(__builtin_amdgcn_ds_bpermute(pa + 4 * k, __float_as_int(wk)));
+                const int off = __builtin_amdgcn_ds_bpermute(pa + 4 * k, ok);
+                if (w != 0.f) {  // the same for the whole row
+                    float t[RPL];
+                    yl_load<RPL>(uv + off, t);
+#pragma unroll
+                    for (int i = 0; i < RPL; i++) acc[i] = fmaf(t[i], w, acc[i]);
+                }
+            }
+        }
+    }
+    const size_t base = (a.gy + row0) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
+    float ov[RPL], o[RPL];
+    yl_load<RPL>(a.out + base, ov);
+#pragma unroll
+    for (int i = 0; i < RPL; i++) {
+        float val = a.accumulate ? ov[i] + acc[i] : acc[i];
+        if (a.truncate && val < 0.f) val = 0.f;
+        o[i] = (row0 + i < a.ny) ? val : ov[i];
+    }
+    yl_store<RPL>(a.out + base, o);
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(256) k_yl_pull_adj(const EtPullArgs *__restrict__ args,
+                                                     const int4 *__restrict__ tab, int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtPullArgs a = ((ET_CONST_AS const EtPullArgs *)args)[j];
+    yl_pull_adj_body<RPL>(a, b);
+}
+
 template <int RPL, int EPI>
 __global__ void __launch_bounds__(256) k_yl_pull(const EtPullArgs *__restrict__ args, const int4 *__restrict__ tab,
                                                  int n)
