@@ -853,9 +853,10 @@ __device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
 // sweep kernels' adjoint pull, so each element sees the same FMA sequence).
 // The candidates lie in a box of at most 3 x 3 columns (bounded through the
 // inverse map); lane k of the row computes candidate k's weight and offset,
-// and the row takes them in turn by cross-lane permutes.  Stores as
-// ET_EPI_YF_STORE (accumulate, truncate; rows below ny).  Block: 16 image
-// columns, column blocks contiguous per XCD.  256 threads.
+// the row's non-zero ones are listed in LDS in candidate order (ballot, rank by
+// popcount), and the row takes them four at a time, the four loads in flight
+// together.  Stores as ET_EPI_YF_STORE (accumulate, truncate; rows below ny).
+// Block: 16 image columns, column blocks contiguous per XCD.  256 threads.
 // ------------------------------------------------------------------------------
 template <int RPL>
 __device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk &b)
@@ -868,7 +869,9 @@ __device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk 
     constexpr int rows = 16 * RPL;  // == a.yu
     const int row0 = RPL * lane;
     const float fxo = (float)x, fzo = (float)z;
-    const int pa = ((int)threadIdx.x & ~15) * 4;  // permute address of lane 0 of this row
+    __shared__ float hit_w[16][16];  // per row: the weights and offsets of its non-zero candidates
+    __shared__ int hit_o[16][16];
+    const int rsh = (int)threadIdx.x & 48;  // this row's first lane within the wave
     float acc[RPL];
 #pragma unroll
     for (int i = 0; i < RPL; i++) acc[i] = 0.f;
@@ -900,15 +903,28 @@ __device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk 
                 wk = wx * wz;
                 ok = rows * ((xc + 1) + a.xp * (zc + 1));
             }
-            const int m = min(16, nc - k0);
-            for (int k = 0; k < m; k++) {
-                const float w = __int_as_float(__builtin_amdgcn_ds_bpermute(pa + 4 * k, __float_as_int(wk)));
-                const int off = __builtin_amdgcn_ds_bpermute(pa + 4 * k, ok);
-                if (w != 0.f) {  // the same for the whole row
-                    float t[RPL];
-                    yl_load<RPL>(uv + off, t);
+            const unsigned mask =
+                (unsigned)(__builtin_amdgcn_ballot_w64(wk != 0.f) >> rsh) & 0xffffu;  // the row's hits
+            if (wk != 0.f) {
+                const int rank = __builtin_popcount(mask & ((1u << lane) - 1u));
+                hit_w[slot][rank] = wk;
+                hit_o[slot][rank] = ok;
+            }
+            const int nh = __builtin_popcount(mask);  // the same for the whole row
+            for (int j = 0; j < nh; j += 4) {
+                float w[4], t[4][RPL];
 #pragma unroll
-                    for (int i = 0; i < RPL; i++) acc[i] = fmaf(t[i], w, acc[i]);
+                for (int q = 0; q < 4; q++) {
+                    const int h = j + q < nh ? j + q : j;  // past the list: reload hit j, weight unused
+                    w[q] = hit_w[slot][h];
+                    yl_load<RPL>(uv + hit_o[slot][h], t[q]);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (j + q < nh) {
+#pragma unroll
+                        for (int i = 0; i < RPL; i++) acc[i] = fmaf(t[q][i], w[q], acc[i]);
+                    }
                 }
             }
         }
