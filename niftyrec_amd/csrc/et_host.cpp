@@ -1234,7 +1234,7 @@ struct YlTable {
                 bt.add(gx, gy);
                 const et_plan *q = plans[i];
                 // tiled projector (key NR 0, R > 0), full-width sweeps, backprojection
-                const bool tiled = kind == YL_PROJECT && std::get<2>(g.first) == 0 && q->R > 0;
+                const bool tiled = (kind == YL_PROJECT || kind == YL_GRAD) && std::get<2>(g.first) == 0 && q->R > 0;
                 lds = std::max(lds, kind == YL_BPU ? q->yl_lds_bpu : (tiled ? q->yl_lds_tile : q->yl_lds_full));
             }
             const auto &k = g.first;
@@ -1345,7 +1345,7 @@ void yl_issue(const float *dev, const std::vector<YlLaunch> &L, size_t from, siz
         case YL_GRAD: {
             ProfScope ps(prof, l.kind == YL_PROJECT ? ET_PROF_PROJECT : (l.kind == YL_BPU ? ET_PROF_BPU : ET_PROF_GRAD),
                          l.bytes);
-            const int k = l.kind == YL_BPU ? 1 : (l.kind == YL_GRAD ? 3 : (l.NR == 0 ? 2 : 0));
+            const int k = l.kind == YL_BPU ? 1 : (l.kind == YL_GRAD ? (l.NR == 0 ? 4 : 3) : (l.NR == 0 ? 2 : 0));
             check_launch(et_launch_yl_sweep(k, l.R, l.RPL, l.NR, l.nthr, l.lds, static_cast<const EtSweepArgs *>(args),
                                             tab, l.n, l.blocks, s));
             break;
@@ -1513,8 +1513,8 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
 }
 
 // Exact attenuation gradient: per view batch, the projection's partial sums
-// (full-width projector, the gradient sweep's chunking), the gradient sweep
-// and the exact adjoint pull.
+// (the plan's projector kind, the gradient sweep's chunking), the gradient
+// sweep of the same kind (tiled or full width) and the exact adjoint pull.
 bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const float *mu, const float *mu_yf,
                     float *out, int truncate)
 {
@@ -1527,17 +1527,18 @@ bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const fl
     float *acc = p->op_acc.ensure(p->yf_elems, true, p->st);
     // every batch in one table (the batches reuse part and u in stream order);
     // the partial sums take the most chunks any batch uses
+    const int tiles = p->yl_tiles;
     int Dmax = 1;
     for (int v0 = 0; v0 < p->V; v0 += vb) {
         int Deff, zlen;
-        yl_chunks(p, yl_op_chunks(p, std::min(vb, p->V - v0)), Deff, zlen);
+        yl_chunks(p, yl_op_chunks(p, (long)std::min(vb, p->V - v0) * tiles), Deff, zlen);
         Dmax = std::max(Dmax, Deff);
     }
     float *part = p->part.ensure((size_t)Dmax * vb * pe, false);
     YlTable t;
     for (int v0 = 0; v0 < p->V; v0 += vb) {
         const int nv = std::min(vb, p->V - v0);
-        const int D = yl_op_chunks(p, nv);
+        const int D = yl_op_chunks(p, (long)nv * tiles);
         const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
         auto args = [&](int &gx, int &gy, double &bytes) {
             bytes += (double)nv * (8.0 * p->nvox + 4.0 * p->P);
@@ -1546,13 +1547,14 @@ bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const fl
             a.part = part;
             a.sino = st + (size_t)v0 * pe;
             a.u = p->u.p;
-            gx = a.D;
+            gx = a.D * tiles;
             gy = nv;
             return a;
         };
-        t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::sweep_key,
+        // the tiled kernels where the plan projects with them (key NR 0)
+        t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::project_key,
                                [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
-        t.grouped<EtSweepArgs>(YL_GRAD, {p}, YlTable::sweep_key,
+        t.grouped<EtSweepArgs>(YL_GRAD, {p}, YlTable::project_key,
                                [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
         yl_pull_into(t, p, recs, nv, true, v0 > 0, v0 + nv >= p->V && truncate, acc);
     }

@@ -177,8 +177,9 @@ int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_
 // (tab as et_launch_sweep_group): sweeps kind 0 projection, 1 rotated-frame
 // backprojection, 3 exact attenuation gradient (reads the projection's partial
 // sums of the same chunking), gx = depth chunks, gy = views, nthr = 16 * W
-// threads, `lds` bytes of LDS; kind 2 the tiled projection (R > 0), gx = depth chunks x
-// tiles (tile fastest), 512 threads, NR unused; pull gx = ceil(nx * nz / 16),
+// threads, `lds` bytes of LDS; kinds 2 and 4 the tiled projection and the tiled
+// gradient (R > 0), gx = depth chunks x tiles (tile fastest), 512 threads, NR
+// unused; pull gx = ceil(nx * nz / 16),
 // gy = 1, 256 threads.  RPL (rows per lane) in {2, 4, 6, 8}, NR (column
 // rounds) in {1, 2, 3}.
 // Tiled row-lane projector: columns per tile, LDS bytes of a block (two

@@ -570,17 +570,23 @@ static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_
 }
 
 // Row-lane sweeps (et_yl.h) of PSF radius R: kind 0 projection, 1 rotated-frame
-// backprojection, 2 tiled projection, 3 exact attenuation gradient; a grouped launch of `blocks` blocks of nthr = 16 * W threads
+// backprojection, 2 tiled projection, 3 exact attenuation gradient, 4 its tiled
+// variant; a grouped launch of `blocks` blocks of nthr = 16 * W threads
 // with `lds` bytes of dynamic LDS (two planes of the widest problem).
 template <int R>
 static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args, const int4 *tab,
                        int n, int blocks, hipStream_t st)
 {
-    if (kind == 2) {  // tiled projector (R > 0)
+    if (kind == 2 || kind == 4) {  // tiled projector, tiled attenuation gradient (R > 0)
 #define ET_YLT(rpl)                                                                                               \
         if (RPL == rpl) {                                                                                         \
-            allow_lds(k_yl_project_tile<R, rpl>, lds);                                                            \
-            hipLaunchKernelGGL((k_yl_project_tile<R, rpl>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);     \
+            if (kind == 2) {                                                                                      \
+                allow_lds(k_yl_project_tile<R, rpl>, lds);                                                        \
+                hipLaunchKernelGGL((k_yl_project_tile<R, rpl>), dim3(blocks), dim3(nthr), lds, st, args, tab, n); \
+            } else {                                                                                              \
+                allow_lds(k_yl_grad_tile<R, rpl>, lds);                                                           \
+                hipLaunchKernelGGL((k_yl_grad_tile<R, rpl>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);    \
+            }                                                                                                     \
             return (int)hipGetLastError();                                                                        \
         }
         if constexpr (R > 0) {

@@ -620,6 +620,124 @@ __device__ __forceinline__ void yl_project_tile_body(const EtSweepArgs &a, const
 }
 
 // ------------------------------------------------------------------------------
+// Tiled exact attenuation gradient: yl_grad_body's reverse sweep on the tiled
+// projector's blocks (tile, d, v) of 512 threads, two per CU; the later
+// chunks' sums come from the tiled projector's partial sums of the same
+// chunking.  Halo columns are blended only; the x' pass runs first, then the
+// axial pass on the owned columns.
+// ------------------------------------------------------------------------------
+template <int R, int RPL>
+__device__ __forceinline__ void yl_grad_tile_body(const EtSweepArgs &a, const Blk &b)
+{
+    constexpr int K = 2 * R + 1, W = 32, NR = 2, NRR = NR + 1;
+    static_assert(W * NR == YL_TILE && 2 * R <= W, "tile shape");
+    extern __shared__ float lds[];
+    const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const int T = a.yl_tiles;
+    const int2 o = xcd_order(b, T, a.D * a.V, b.x % T, b.x / T + a.D * b.y);
+    const int tile = o.x, d = o.y % a.D, v = o.y / a.D;
+    const int x_lo = tile * YL_TILE;
+    const int zb = d * a.zlen, ze = min(a.nz, zb + a.zlen);
+    constexpr int rows = 16 * RPL, CS = rows + 4;  // == a.yu
+    constexpr int PF = (YL_TILE + 2 * R) * CS;     // LDS column c holds column x_lo - R + c
+    const int row0 = RPL * lane;
+    const int dx = a.yp, dz = a.yp * a.xp;
+    const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
+    yl_zero_lds(lds, 2 * PF);
+    const float *act = a.act + a.gy + row0;
+    float *uv = a.u + (size_t)v * a.ustride + row0;
+    auto col = [&](int r) -> int {
+        if (r < NR) return x_lo + r * W + slot;
+        if (slot < R) return x_lo - R + slot;
+        return slot < 2 * R ? x_lo + YL_TILE + slot - R : -1;
+    };
+    auto valid = [&](int xc) { return xc >= 0 && xc < a.nx; };
+
+    float acc[NR][RPL], cmp[NR][RPL];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int xc = col(r);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            acc[r][i] = 0.f;
+            cmp[r][i] = 0.f;
+        }
+        if (!valid(xc)) continue;
+        float hi[RPL], lo[RPL];
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            hi[i] = 0.f;
+            lo[i] = 0.f;
+        }
+        for (int d2 = d + 1; d2 < a.D; d2++) {
+            float t[RPL];
+            yl_load<RPL>(a.part + (((size_t)d2 * a.V + v) * a.nx + xc) * rows + row0, t);
+#pragma unroll
+            for (int i = 0; i < RPL; i++) comp_add(hi[i], lo[i], t[i]);
+        }
+        if (d == a.D - 1 && a.bg_line != 0.f) {
+            float e[RPL];
+            yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + a.nz - 1) * a.nx + xc) * rows + row0, e);
+#pragma unroll
+            for (int i = 0; i < RPL; i++) comp_add(hi[i], lo[i], a.bg_line * e[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            acc[r][i] = hi[i];
+            cmp[r][i] = -lo[i];
+        }
+    }
+    auto kahan = [&](int r, int i, float h, float e) {
+        const float y = fmaf(h, e, -cmp[r][i]);
+        const float t = acc[r][i] + y;
+        cmp[r][i] = (t - acc[r][i]) - y;
+        acc[r][i] = t;
+    };
+
+    constexpr int PB = 16 / NRR;  // planes per sample batch (descending)
+    YlSamp sb;
+    int kb = 0;
+    for (int z = ze - 1; z >= zb; z--, kb = kb + 1 == PB ? 0 : kb + 1) {
+        const float *kz = psf_plane<R>(a.psf, z);
+        if (kb == 0)  // (planes before the chunk and absent columns give samples no round takes)
+            sb = yl_samp_of(fast_sample(fv, (float)col(lane % NRR), z - lane / NRR, a.nx, a.nz, dx, dz));
+        float e[NR][RPL];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = col(r);
+            if (valid(xc)) yl_load_stream<RPL>(a.att + (((size_t)v * a.nz + z) * a.nx + xc) * rows + row0, e[r]);
+        }
+        float *plane = lds + (z & 1) * PF;
+#pragma unroll
+        for (int r = 0; r < NRR; r++) {
+            const int xc = col(r);
+            if (valid(xc)) {
+                const Samp s = yl_samp_from(sb, kb * NRR + r);
+                float bl[RPL];
+                yl_bilinear<RPL>(act + s.off, dx, dz, s, bl);
+                lds_store<RPL>(plane + (xc - x_lo + R) * CS + row0, bl);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const int xc = col(r);
+            if (valid(xc)) {
+                float t[RPL], h[RPL], sv[RPL], out[RPL];
+                yl_row_pass<R, RPL>(plane, CS, xc - x_lo, row0, kz, t);
+                yl_column_pass<R, RPL>(t, kz + K, h);
+#pragma unroll
+                for (int i = 0; i < RPL; i++) kahan(r, i, h[i], e[r][i]);
+                yl_load<RPL>(a.sino + ((size_t)v * a.nx + xc) * rows + row0, sv);
+#pragma unroll
+                for (int i = 0; i < RPL; i++) out[i] = -sv[i] * (acc[r][i] - cmp[r][i]);
+                yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), out);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------
 // Rotated-frame backprojection with cached attenuation factors:
 // u(x', y, z') = PSF_z'( s(x', y) e^{-A(x', y, z')} ), s the ratio [v][x'][rows]
 // (ET_SRC_SINO) or 1 on the rows below ny (ET_SRC_ONES).  Reference
@@ -697,6 +815,17 @@ __global__ void __launch_bounds__(1024) k_yl_project(const EtSweepArgs *__restri
     if (!group_block(tab, n, b, j)) return;
     const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
     yl_project_body<R, RPL, NR>(a, b);
+}
+
+template <int R, int RPL>
+__global__ void __launch_bounds__(512, 4) k_yl_grad_tile(const EtSweepArgs *__restrict__ args,
+                                                       const int4 *__restrict__ tab, int n)
+{
+    Blk b;
+    int j;
+    if (!group_block(tab, n, b, j)) return;
+    const EtSweepArgs a = ((ET_CONST_AS const EtSweepArgs *)args)[j];
+    yl_grad_tile_body<R, RPL>(a, b);
 }
 
 // Two blocks per CU: the second launch-bounds argument is the minimum number
