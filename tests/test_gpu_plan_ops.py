@@ -2,9 +2,9 @@
 row-lane kernels: projection, reference and exact-adjoint backprojection and
 the exact attenuation gradient, against the oracle (single and double
 precision) at widths that take the full-width and the tiled projector; the
-cached attenuation factors follow the map's content; and an OSEM call after
-operations that reallocate the plan's work buffers (its cached launch table
-is then rebuilt) matches a plan that ran OSEM alone."""
+cached attenuation factors follow the map's content; and OSEM (a plan's or a
+batch's) after operations that reallocate a plan's work buffers (the cached
+launch table or graph is then rebuilt) matches runs without them."""
 import numpy as np
 import pytest
 
@@ -100,3 +100,29 @@ def test_osem_after_operations_that_grow_the_buffers():
             x32 = orc.mlem_step(x32, y[s::4], mu, cams[s::4], psf)
             x64 = orc.mlem_step(x64, y[s::4], mu, cams[s::4], psf, dtype=np.float64)
     report_parity("osem after operations that grow the buffers", X[0].cpu().numpy(), x32, x64)
+
+
+def test_batch_after_operations_that_grow_the_buffers():
+    """A batch replays a captured graph that refers to its plans' work buffers;
+    after a single operation of one plan reallocates them, the next batch call
+    captures again: bit-identical to a batch whose plans ran nothing else."""
+    from niftyrec_amd.api import Batch, Plan
+    cases = [_case((48, 20, 48), 16, 5, seed=94), _case((40, 24, 40), 12, 3, seed=95)]
+    ys = [orc.project(act, mu, cams, psf) for act, mu, cams, psf in cases]
+    runs = []
+    for grow in (True, False):
+        plans = [Plan(act.shape, cams, psf) for act, mu, cams, psf in cases]
+        for p in plans:
+            p.set_tuning(4, 0)
+        batch = Batch(plans)
+        X = [torch.ones(c[0].shape, dtype=torch.float32, device="cuda") for c in cases]
+        Y = [_t(y) for y in ys]
+        M = [_t(c[1]) for c in cases]
+        batch.osem(X, Y, M, subsets=4, iterations=1)
+        if grow:
+            plans[0].backproject(Y[0], M[0])  # 16 views of rotated-frame volumes, against OSEM's 4
+        batch.osem(X, Y, M, subsets=4, iterations=1)
+        torch.cuda.synchronize()
+        runs.append([x.clone() for x in X])
+    for i in range(len(cases)):
+        assert torch.equal(runs[0][i], runs[1][i]), "problem %d" % i
