@@ -34,7 +34,8 @@ def _t(a):
 
 
 @pytest.mark.parametrize("shape, V, K", [((40, 20, 40), 10, 5), ((32, 24, 32), 8, 0), ((96, 20, 96), 8, 7),
-                                          ((130, 24, 130), 6, 11)])
+                                          ((130, 24, 130), 6, 11), ((17, 1, 17), 3, 3), ((65, 7, 65), 1, 5),
+                                          ((23, 45, 23), 5, 7)])
 def test_plan_operations_on_row_lane_kernels(shape, V, K):
     from niftyrec_amd.api import ADJOINT, Plan
     act, mu, cams, psf = _case(shape, V, K, seed=90)
