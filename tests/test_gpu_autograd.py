@@ -210,6 +210,24 @@ def test_plan_osem_wide_volumes(shape, V, K):
     report_parity("plan osem %s, V=%d, K=%d" % (shape, V, K), X.cpu().numpy(), x, x64)
 
 
+@pytest.mark.parametrize("shape, V, K", [((24, 140, 24), 8, 5), ((200, 10, 200), 4, 7)])
+def test_plan_osem_sweep_kernels(shape, V, K):
+    """OSEM through a plan the row-lane kernels do not take (more than 128
+    axial rows, or more than 192 columns): the sweep kernels with cached
+    attenuation factors, timed depth chunks."""
+    act, mu, cams, psf = _case(shape=shape, V=V, K=K, seed=16)
+    plan = _plan(act.shape, cams, psf)
+    y = orc.project(act, mu, cams, psf)
+    X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
+    plan.osem(X, _t(y), _t(mu), subsets=2, iterations=1)
+    assert plan.projector() == 0  # the sweep kernels
+    x, x64 = np.ones_like(act), np.ones(act.shape)
+    for s in range(2):
+        x = orc.mlem_step(x, y[s::2], mu, cams[s::2], psf)
+        x64 = orc.mlem_step(x64, y[s::2], mu, cams[s::2], psf, dtype=np.float64)
+    report_parity("plan osem on the sweep kernels %s, V=%d, K=%d" % (shape, V, K), X.cpu().numpy(), x, x64)
+
+
 def test_plan_osem_general_path():
     """A plan with a non-zero background takes the materialising path: OSEM
     with one subset equals the oracle's MLEM steps; more subsets are refused
