@@ -1787,22 +1787,20 @@ void osem_prepare(et_plan *p, float *x, const float *meas, const float *mu, int 
         p->osem_Db = Db;
         p->osem_mu_yf = mu_yf != nullptr;
         p->sens.ensure((size_t)S * p->yf_elems, true, p->st);
-        // attenuation factors of every view, once per attenuation map: a
-        // projection sweep (of mu itself; its sums are discarded) stores them
+        // attenuation factors of every view, once per attenuation map
         if (mu_yf) p->osem_E.ensure(att_elems(p, p->V), false);
-        const int Dm = std::max(1, std::min(8, p->nz / 16));
         for (int s = 0; s < S; s++) {
             const int nv = start[s + 1] - start[s];
             const float *r = p->osem_recs.p + (size_t)start[s] * ET_VIEW_REC;
             float *E = osem_att(p, start[s]);
             if (mu_yf) {
-                const float *as = nullptr;
-                if (Dm > 1) {
-                    mu_prefix(p, r, nv, Dm, mu_yf, true, false);
-                    as = p->astart.p;
-                }
-                float *parts = p->part.ensure((size_t)Dm * nv * p->P, false);
-                project_fast(p, r, nv, mu_yf, mu_yf, nullptr, 0, Dm, as, parts, E, false);
+                EtSweepArgs a = base_args(p);
+                a.views = r;
+                a.V = nv;
+                a.mu = mu_yf;
+                a.att = E;
+                ProfScope ps(p, ET_PROF_OTHER, (double)nv * 8.0 * p->nvox);
+                check_launch(et_launch_att_factors(a, (et_stream_t)p->st));
             }
             em_sensitivity(p, r, nv, mu_yf, eps, Db, E, p->sens.p + (size_t)s * p->yf_elems);
         }

@@ -193,6 +193,9 @@ int et_launch_yl_sweep(int kind, int R, int RPL, int NR, int nthr, size_t lds, c
                        const void *tab, int n, int blocks, et_stream_t stream);
 int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void *tab, int n, int blocks,
                       et_stream_t stream);
+// Attenuation factors of a.V views for the sweep kernels' OSEM cache: a.att
+// [v][z'][yu][roundup(nx, 64)] = e^{-A} from a.mu (YF); 16-row slabs.
+int et_launch_att_factors(const EtSweepArgs &a, et_stream_t stream);
 // Exact adjoint pull of the row-lane rotated-frame volumes into a.out (YF layout,
 // accumulate / truncate as ET_EPI_YF_STORE); grid as the row-lane pull.
 int et_launch_yl_pull_adj(int RPL, const EtPullArgs *args, const void *tab, int n, int blocks, et_stream_t stream);

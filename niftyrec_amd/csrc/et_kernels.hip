@@ -1020,6 +1020,15 @@ int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void 
     return (int)hipErrorInvalidValue;
 }
 
+int et_launch_att_factors(const EtSweepArgs &a, et_stream_t stream)
+{
+    const int nthr = ((a.nx + 63) / 64) * 64;
+    if (nthr > 512 || a.V <= 0) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_att_factors<16>, dim3((a.ny + 15) / 16, a.V), dim3(nthr), 0,
+                       reinterpret_cast<hipStream_t>(stream), a);
+    return (int)hipGetLastError();
+}
+
 int et_launch_yl_pull_adj(int RPL, const EtPullArgs *args, const void *tab, int n, int blocks, et_stream_t stream)
 {
     if (n <= 0 || blocks <= 0) return (int)hipErrorInvalidValue;

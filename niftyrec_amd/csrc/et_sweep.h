@@ -395,6 +395,31 @@ __global__ void __launch_bounds__(512) k_grad_mu(EtSweepArgs a)
     }
 }
 
+// Attenuation factors of the sweep kernels' OSEM cache, att[v][z'][yu][nthr] =
+// e^{-A(x', y, z')}, A the compensated (TwoSum) inclusive sum of the resampled
+// mu along depth from the detector: what the AM = 1 projection stores, without
+// its projection.  Block: one slab of TS rows of one view, thread = rotated
+// column; grid (slabs, V).
+template <int TS>
+__global__ void __launch_bounds__(512) k_att_factors(EtSweepArgs a)
+{
+    const SweepCtx c = sweep_ctx<TS, 0>(a, false, blk_here());
+    float A[TS], Al[TS];
+#pragma unroll
+    for (int i = 0; i < TS; i++) {
+        A[i] = 0.f;
+        Al[i] = 0.f;
+    }
+    for (int z = 0; z < a.nz; z++) {
+        const Samp s = fast_sample(c.fv, c.fxp, z, a.nx, a.nz, c.dx, c.dz);
+        bilinear_map<TS, TS, 0>(a.mu + s.off + (a.gmu + c.y0), c.dx, c.dz, s,
+                                [&](int i, float m) { comp_add(A[i], Al[i], m); });
+        float *ez = a.att + (((size_t)c.v * a.nz + z) * a.yu + c.y0) * c.nthr + c.tx;
+#pragma unroll
+        for (int i = 0; i < TS; i++) gstore(ez + i * c.nthr, exp_neg(A[i] + Al[i]));
+    }
+}
+
 // MUSUM: part[v][d][y][x] = sum over chunk d of mu'(x, y, z'), accumulated as
 // a compensated (hi, lo) pair; the low parts go to part_lo (same layout).
 template <int TS>
