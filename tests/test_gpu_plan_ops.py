@@ -59,6 +59,33 @@ def test_plan_operations_on_row_lane_kernels(shape, V, K):
                   orc.gradient_mu(y, act, mu, cams, psf, dtype=np.float64))
 
 
+def test_plan_operations_in_view_batches():
+    """Backprojections and the gradient in batches of 3 views (set_tuning's
+    view batch): the pulls accumulate over the batches and truncate after the
+    last, and the launches of every batch go in one table.  The gradient takes
+    non-negative weights, as the other gradient tests: with signed ones its
+    tails cancel, and there the FP32 results (this library's and the oracle's
+    alike) are up to 3e-3 from FP64 element-wise."""
+    from niftyrec_amd.api import ADJOINT, Plan
+    act, mu, cams, psf = _case((40, 20, 40), 10, 5, seed=96)
+    plan = Plan(act.shape, cams, psf)
+    plan.set_tuning(0, 3)
+    A, M = _t(act), _t(mu)
+    y = np.random.default_rng(97).random((10,) + act.shape[1:], dtype=np.float32) - np.float32(0.3)
+    b = plan.backproject(_t(y), M).cpu().numpy()  # negative values: truncation at the end matters
+    report_parity("plan backproject, reference, 3-view batches", b, orc.backproject(y, mu, cams, psf),
+                  orc.backproject(y, mu, cams, psf, dtype=np.float64))
+    ba = plan.backproject(_t(y), M, mode=ADJOINT, truncate=False).cpu().numpy()
+    report_parity("plan backproject, exact adjoint, 3-view batches", ba,
+                  orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1),
+                  orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1, dtype=np.float64))
+    s = np.abs(y)
+    g = plan.gradient_attenuation(_t(s), A, M, exact=True, truncate=False).cpu().numpy()
+    report_parity("plan exact attenuation gradient, 3-view batches", g,
+                  orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float32),
+                  orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float64))
+
+
 def test_plan_operation_factors_follow_the_map():
     """The factors cached for the single operations are keyed on the map's
     content: a map rewritten in place (same address) is caught."""
