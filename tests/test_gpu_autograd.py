@@ -210,7 +210,7 @@ def test_plan_osem_wide_volumes(shape, V, K):
     report_parity("plan osem %s, V=%d, K=%d" % (shape, V, K), X.cpu().numpy(), x, x64)
 
 
-@pytest.mark.parametrize("shape, V, K", [((24, 140, 24), 8, 5), ((200, 10, 200), 4, 7)])
+@pytest.mark.parametrize("shape, V, K", [((24, 140, 24), 8, 5), ((200, 10, 200), 4, 7), ((40, 140, 40), 6, 17)])
 def test_plan_osem_sweep_kernels(shape, V, K):
     """OSEM through a plan the row-lane kernels do not take (more than 128
     axial rows, or more than 192 columns): the sweep kernels with cached
