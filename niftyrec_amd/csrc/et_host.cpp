@@ -279,10 +279,17 @@ struct et_plan {
     DevBuf fac, sino_t, op_tables, op_acc;  // (op_acc: backprojection accumulator, YF layout)
     bool fac_valid = false;
     unsigned long long fac_key = 0;
+    // the last projection's partial sums [fwd_D][V][x'][rows] and the content
+    // keys of its activity and map: an exact gradient of the same pair seeds
+    // its reverse sweeps from them instead of projecting again
+    DevBuf fwd_part;
+    bool fwd_valid = false;
+    unsigned long long fwd_act = 0, fwd_mu = 0;
+    int fwd_D = 0, fwd_tiles = 0;  // (fwd_D: the depth chunks the projection asked yl_chunks for)
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
     std::vector<const void *> osem_sig;                // buffers yl_launches refer to
-    DevBuf fp_dev;                  // fingerprint (8 bytes)
-    unsigned long long *fp_host = nullptr;  // pinned copy of the fingerprint
+    DevBuf fp_dev;                  // fingerprints (two of 8 bytes)
+    unsigned long long *fp_host = nullptr;  // pinned copies of the fingerprints
     // tuning
     int tune_D = 0, tune_vb = 0;
     // profiling
@@ -811,11 +818,12 @@ void pull_general(et_plan *p, int v, const float *u, bool adjoint, bool accumula
 
 // Row-lane versions (defined with the row-lane kernels' host code); false when
 // the plan does not take them.
-bool yl_op_project(et_plan *p, const float *act_yf, const float *mu, const float *mu_yf, float *sino, int truncate);
+bool yl_op_project(et_plan *p, const float *act, const float *act_yf, const float *mu, const float *mu_yf, float *sino,
+                   int truncate);
 bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const float *mu_yf, float *out, bool adjoint,
                        int truncate);
-bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const float *mu, const float *mu_yf,
-                    float *out, int truncate);
+bool yl_op_gradient(et_plan *p, const float *sino, const float *act, const float *act_yf, const float *mu,
+                    const float *mu_yf, float *out, int truncate);
 
 void op_project(et_plan *p, const float *act, const float *mu, float *sino, int truncate)
 {
@@ -836,7 +844,7 @@ void op_project(et_plan *p, const float *act, const float *mu, float *sino, int 
                                            (et_stream_t)p->st));
             return;
         }
-        if (yl_op_project(p, act_yf, mu, mu_yf, sino, truncate)) return;
+        if (yl_op_project(p, act, act_yf, mu, mu_yf, sino, truncate)) return;
         const int D = mu ? choose_D(p, p->V) : std::max(1, choose_D(p, p->V));
         const float *astart = nullptr;
         if (mu && D > 1) {
@@ -928,7 +936,7 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
     if (p->fast) {
         pack(p, act, p->act_yf);
         pack(p, mu, p->mu_yf);
-        if (exact && yl_op_gradient(p, sino, p->act_yf.p, mu, p->mu_yf.p, out, truncate)) return;
+        if (exact && yl_op_gradient(p, sino, act, p->act_yf.p, mu, p->mu_yf.p, out, truncate)) return;
         const int vb = grad_view_batch(p);
         p->u.ensure((size_t)vb * p->ustride, true, p->st);
         for (int v0 = 0; v0 < p->V; v0 += vb) {
@@ -1160,9 +1168,9 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
 void mu_key_launch(et_plan *p, const float *mu, bool content = false)
 {
     if (!mu || !(p->mu_check || content)) return;
-    p->fp_dev.ensure(2, false);
+    p->fp_dev.ensure(4, false);
     if (!p->fp_host &&
-        hipHostMalloc((void **)&p->fp_host, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&p->fp_host, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
         p->fp_host = nullptr;
         throw EtError{niftyrec_error_alloccpu};
     }
@@ -1386,15 +1394,29 @@ void yl_upload(DevBuf &buf, const YlTable &t, hipStream_t st)
 // projector, and the rotated-frame volumes go to the sweep kernels' pulls
 // (same u layout).
 
-// Factors of every view in view order ([v][z'][x'][rows]) for the map mu (mu_yf
-// its YF copy), rebuilt when the map's content fingerprint changes (whatever
-// the plan's mu_check, which keys the OSEM caches: these calls may see other
-// maps between OSEM calls, and a map rewritten in place is caught).
-const float *yl_view_factors(et_plan *p, const float *mu, const float *mu_yf)
+// Content fingerprints of the map mu and (when given) the activity act, with
+// one stream synchronisation.  The single operations key their caches on
+// content whatever the plan's mu_check (which keys the OSEM caches): they may
+// see other maps between OSEM calls, and a map rewritten in place is caught.
+void content_keys(et_plan *p, const float *mu, const float *act, unsigned long long &kmu, unsigned long long &kact)
 {
     mu_key_launch(p, mu, true);
+    if (act) {
+        ProfScope ps(p, ET_PROF_OTHER, 4.0 * p->nvox);
+        unsigned long long *d = reinterpret_cast<unsigned long long *>(p->fp_dev.p) + 1;
+        check_launch(et_launch_fingerprint(act, p->nvox, d, (et_stream_t)p->st));
+        check_hip(hipMemcpyAsync(p->fp_host + 1, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->st),
+                  niftyrec_error_transfergpu);
+    }
     check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
-    const unsigned long long key = mu_key_read(p, mu, true);
+    kmu = mu_key_read(p, mu, true);
+    kact = act ? p->fp_host[1] : 0;
+}
+
+// Factors of every view in view order ([v][z'][x'][rows]) for the map mu (mu_yf
+// its YF copy; key its content key), rebuilt when the key changes.
+const float *yl_view_factors(et_plan *p, const float *mu_yf, unsigned long long key)
+{
     if (p->fac_valid && p->fac_key == key) return p->fac.p;
     p->fac_valid = false;
     p->fac.ensure(yl_att_elems(p, p->V), false);
@@ -1453,14 +1475,18 @@ void yl_pull_into(YlTable &t, et_plan *p, const float *recs, int nv, bool adjoin
                           });
 }
 
-bool yl_op_project(et_plan *p, const float *act_yf, const float *mu, const float *mu_yf, float *sino, int truncate)
+bool yl_op_project(et_plan *p, const float *act, const float *act_yf, const float *mu, const float *mu_yf, float *sino,
+                   int truncate)
 {
     if (!p->yl_ops || !p->yl || !act_yf || !mu) return false;
-    const float *fac = yl_view_factors(p, mu, mu_yf);
+    unsigned long long kmu, kact;
+    content_keys(p, mu, act, kmu, kact);
+    const float *fac = yl_view_factors(p, mu_yf, kmu);
     const int D = yl_op_chunks(p, (long)p->V * p->yl_tiles);
     int Deff, zlen;
     yl_chunks(p, D, Deff, zlen);
-    float *part = p->part.ensure((size_t)Deff * p->V * yl_plane_elems(p), false);
+    p->fwd_valid = false;
+    float *part = p->fwd_part.ensure((size_t)Deff * p->V * yl_plane_elems(p), false);
     YlTable t;
     t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::project_key, [&](int, int &gx, int &gy, double &bytes) {
         bytes += (double)p->V * (8.0 * p->nvox + 4.0 * p->P);
@@ -1475,6 +1501,11 @@ bool yl_op_project(et_plan *p, const float *act_yf, const float *mu, const float
     ProfScope ps(p, ET_PROF_OTHER, (Deff + 1.0) * 4.0 * p->V * p->P);
     check_launch(
         et_launch_yl_parts_to_sino(part, Deff, p->V, p->nx, p->ny, p->yu, truncate, sino, (et_stream_t)p->st));
+    p->fwd_valid = true;
+    p->fwd_act = kact;
+    p->fwd_mu = kmu;
+    p->fwd_D = D;  // the chunking as requested: yl_chunks maps it to the same chunks again
+    p->fwd_tiles = p->yl_tiles;
     return true;
 }
 
@@ -1482,7 +1513,9 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
                        int truncate)
 {
     if (!p->yl_ops || !p->yl || !mu) return false;
-    const float *fac = yl_view_factors(p, mu, mu_yf);
+    unsigned long long kmu, kact;
+    content_keys(p, mu, nullptr, kmu, kact);
+    const float *fac = yl_view_factors(p, mu_yf, kmu);
     const size_t pe = yl_plane_elems(p);
     const float *st = yl_sino_t(p, sino);
     const int vb = view_batch(p);
@@ -1515,30 +1548,41 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
 // Exact attenuation gradient: per view batch, the projection's partial sums
 // (the plan's projector kind, the gradient sweep's chunking), the gradient
 // sweep of the same kind (tiled or full width) and the exact adjoint pull.
-bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const float *mu, const float *mu_yf,
-                    float *out, int truncate)
+bool yl_op_gradient(et_plan *p, const float *sino, const float *act, const float *act_yf, const float *mu,
+                    const float *mu_yf, float *out, int truncate)
 {
     if (!p->yl_ops || !p->yl) return false;
-    const float *fac = yl_view_factors(p, mu, mu_yf);
+    unsigned long long kmu, kact;
+    content_keys(p, mu, act, kmu, kact);
+    const float *fac = yl_view_factors(p, mu_yf, kmu);
     const size_t pe = yl_plane_elems(p);
     const float *st = yl_sino_t(p, sino);
-    const int vb = view_batch(p);
+    const int tiles = p->yl_tiles;
+    // the last projection's partial sums, when it projected this activity
+    // through this map with this projector: all views in one batch at its
+    // chunking, no projection pass
+    const bool reuse = p->fwd_valid && p->fwd_act == kact && p->fwd_mu == kmu && p->fwd_tiles == tiles;
+    const int vb = reuse ? p->V : view_batch(p);
     p->u.ensure((size_t)vb * p->ustride, true, p->st);
     float *acc = p->op_acc.ensure(p->yf_elems, true, p->st);
     // every batch in one table (the batches reuse part and u in stream order);
     // the partial sums take the most chunks any batch uses
-    const int tiles = p->yl_tiles;
-    int Dmax = 1;
-    for (int v0 = 0; v0 < p->V; v0 += vb) {
-        int Deff, zlen;
-        yl_chunks(p, yl_op_chunks(p, (long)std::min(vb, p->V - v0) * tiles), Deff, zlen);
-        Dmax = std::max(Dmax, Deff);
+    float *part = nullptr;
+    if (reuse) {
+        part = p->fwd_part.p;
+    } else {
+        int Dmax = 1;
+        for (int v0 = 0; v0 < p->V; v0 += vb) {
+            int Deff, zlen;
+            yl_chunks(p, yl_op_chunks(p, (long)std::min(vb, p->V - v0) * tiles), Deff, zlen);
+            Dmax = std::max(Dmax, Deff);
+        }
+        part = p->part.ensure((size_t)Dmax * vb * pe, false);
     }
-    float *part = p->part.ensure((size_t)Dmax * vb * pe, false);
     YlTable t;
     for (int v0 = 0; v0 < p->V; v0 += vb) {
         const int nv = std::min(vb, p->V - v0);
-        const int D = yl_op_chunks(p, (long)nv * tiles);
+        const int D = reuse ? p->fwd_D : yl_op_chunks(p, (long)nv * tiles);
         const float *recs = p->d_recs.p + (size_t)v0 * ET_VIEW_REC;
         auto args = [&](int &gx, int &gy, double &bytes) {
             bytes += (double)nv * (8.0 * p->nvox + 4.0 * p->P);
@@ -1552,8 +1596,9 @@ bool yl_op_gradient(et_plan *p, const float *sino, const float *act_yf, const fl
             return a;
         };
         // the tiled kernels where the plan projects with them (key NR 0)
-        t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::project_key,
-                               [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
+        if (!reuse)
+            t.grouped<EtSweepArgs>(YL_PROJECT, {p}, YlTable::project_key,
+                                   [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
         t.grouped<EtSweepArgs>(YL_GRAD, {p}, YlTable::project_key,
                                [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
         yl_pull_into(t, p, recs, nv, true, v0 > 0, v0 + nv >= p->V && truncate, acc);
@@ -3324,6 +3369,7 @@ int et_plan_invalidate(et_plan *plan)
     if (!plan) return niftyrec_error_parameters;
     plan->osem_valid = false;
     plan->fac_valid = false;
+    plan->fwd_valid = false;
     return niftyrec_success;
 }
 
