@@ -633,12 +633,31 @@ __device__ __forceinline__ unsigned long long fp_mix(unsigned long long h)
     return h;
 }
 
+// The key is a sum modulo 2^64 of per-element hashes: any grid and order give
+// the same value.  Few blocks (one atomic each) and 16-byte loads where p is
+// 16-byte aligned (vec4).
+template <bool VEC4>
 __global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict__ p, size_t n,
                                                      unsigned long long *__restrict__ out)
 {
+    constexpr unsigned long long G = 0x9e3779b97f4a7c15ull;
     unsigned long long h = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        h += fp_mix(((unsigned long long)p[i] << 32) ^ (i * 0x9e3779b97f4a7c15ull));
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    size_t done = 0;
+    if constexpr (VEC4) {
+        const size_t n4 = n / 4;
+        const uint4 *p4 = reinterpret_cast<const uint4 *>(p);
+        for (size_t j = t0; j < n4; j += stride) {
+            const uint4 v = p4[j];
+            const size_t i = 4 * j;
+            h += fp_mix(((unsigned long long)v.x << 32) ^ (i * G)) +
+                 fp_mix(((unsigned long long)v.y << 32) ^ ((i + 1) * G)) +
+                 fp_mix(((unsigned long long)v.z << 32) ^ ((i + 2) * G)) +
+                 fp_mix(((unsigned long long)v.w << 32) ^ ((i + 3) * G));
+        }
+        done = 4 * n4;
+    }
+    for (size_t i = done + t0; i < n; i += stride) h += fp_mix(((unsigned long long)p[i] << 32) ^ (i * G));
     for (int o = 32; o >= 1; o >>= 1) h += __shfl_xor(h, o, 64);
     __shared__ unsigned long long part[4];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h;
@@ -977,9 +996,12 @@ int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_
 {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(out), (size_t)2, 0.f);
-    const unsigned blocks = (unsigned)std::min<size_t>(nblk(n, 256), 2048);
-    hipLaunchKernelGGL(k_fingerprint, dim3(std::max(1u, blocks)), dim3(256), 0, st, reinterpret_cast<const unsigned *>(p),
-                       n, out);
+    const unsigned blocks = std::max(1u, (unsigned)std::min<size_t>(nblk(n, 1024), 512));
+    const unsigned *q = reinterpret_cast<const unsigned *>(p);
+    if ((reinterpret_cast<uintptr_t>(p) & 15) == 0)
+        hipLaunchKernelGGL(k_fingerprint<true>, dim3(blocks), dim3(256), 0, st, q, n, out);
+    else
+        hipLaunchKernelGGL(k_fingerprint<false>, dim3(blocks), dim3(256), 0, st, q, n, out);
     return (int)hipGetLastError();
 }
 
