@@ -272,6 +272,18 @@ int et_plan_backproject(et_plan *plan, const float *sino, const float *mu, float
 int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *act,
                                  const float *mu, float *out, int exact, int truncate,
                                  void *stream);
+/* Both gradients of <sino, project(act, mu)> (sino holds dL/dp): out_act as
+ * et_plan_backproject with adjoint 1, out_mu as et_plan_gradient_attenuation
+ * with exact 1, one truncate flag for both.  No counterpart in the reference:
+ * the backward pass of a differentiable projection.  Right after
+ * et_plan_project of the same act and mu, plans on the row-lane kernels run
+ * both rotated-frame sweeps and one exact adjoint pull whose candidate weights
+ * serve both outputs (rotated-frame volumes of all views twice: at most
+ * 4 GiB); otherwise the two operations in turn.  Outputs must not alias each
+ * other or an input (niftyrec_error_parameters). */
+int et_plan_project_backward(et_plan *plan, const float *sino, const float *act,
+                             const float *mu, float *out_act, float *out_mu, int truncate,
+                             void *stream);
 /* Ordered-subsets EM on the device: subset s holds views s, s+S, s+2S, ...;
  * each subset update is the reference MLEM update restricted to its views.
  * act is updated in place.  The sensitivity of every subset, the attenuation

@@ -71,6 +71,8 @@ SIGNATURES = {
     "et_plan_backproject": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "et_plan_gradient_attenuation": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
                                              c_void_p]),
+    "et_plan_project_backward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                         c_void_p]),
     "et_plan_osem": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_void_p]),
     "et_plan_set_tuning": (c_int, [c_void_p, c_int, c_int]),
     "et_plan_set_mu_check": (c_int, [c_void_p, c_int]),

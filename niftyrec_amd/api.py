@@ -419,6 +419,28 @@ class Plan:
         self._written(out)
         return out
 
+    def project_backward(self, sinogram, activity, attenuation, truncate=False, out_activity=None,
+                         out_attenuation=None):
+        """Both gradients of <sinogram, project(activity, attenuation)>: the exact
+        adjoint backprojection of the sinogram and the exact attenuation
+        gradient, as backproject(mode=ADJOINT) and gradient_attenuation(exact=True)
+        return them; right after project() of the same pair, one pass over the
+        views (et_plan_project_backward).  Returns (d/d activity, d/d attenuation)."""
+        if activity is None or attenuation is None:
+            raise ValueError("project_backward needs the activity and the attenuation map")
+        s = self._arg(sinogram, self.sino_shape, "sinogram")
+        a = self._arg(activity, self.shape, "activity")
+        m = self._arg(attenuation, self.shape, "attenuation")
+        ga = self._new(self.shape, activity) if out_activity is None else out_activity
+        gm = self._new(self.shape, activity) if out_attenuation is None else out_attenuation
+        oa = self._arg(ga, self.shape, "out_activity")
+        om = self._arg(gm, self.shape, "out_attenuation")
+        st = self._L.et_plan_project_backward(self._h, s, a, m, oa, om, int(bool(truncate)), self._stream())
+        _lib.check(st, "et_plan_project_backward")
+        self._written(ga)
+        self._written(gm)
+        return ga, gm
+
     def osem(self, activity, sinogram, attenuation=None, subsets=1, iterations=1, epsilon=1e-6):
         """Ordered-subsets EM in place on `activity` (subset s: views s, s + S, ...).
         The subset sensitivities and attenuation factors are cached in the

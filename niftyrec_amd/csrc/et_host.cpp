@@ -224,14 +224,19 @@ struct DevBuf {
 // One grouped launch of the row-lane OSEM kernels: argument records and block
 // table at byte offsets of a device table buffer.
 struct YlLaunch {
-    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE, YL_GRAD, YL_PULL_ADJ
+    int kind;                 // YL_PROJECT, YL_RATIO, YL_BPU, YL_PULL_OSEM, YL_PULL_STORE, YL_GRAD, YL_PULL_ADJ(2)
     int R, RPL, NR, nthr;     // sweep template and block shape (pull: RPL)
     size_t lds;               // dynamic LDS bytes (sweeps)
     size_t args_off, tab_off;
     int n, blocks;
     double bytes;             // algorithmic bytes (the profiling scopes' figure)
 };
-enum { YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4, YL_GRAD = 5, YL_PULL_ADJ = 6 };
+enum {
+    YL_PROJECT = 0, YL_RATIO = 1, YL_BPU = 2, YL_PULL_OSEM = 3, YL_PULL_STORE = 4, YL_GRAD = 5, YL_PULL_ADJ = 6,
+    YL_PULL_ADJ2 = 7  // exact adjoint pull of two volumes (u, u2) into two outputs
+};
+// Table slots of the single operations (each keeps its last upload).
+enum { YL_OP_PROJECT = 0, YL_OP_BACKPROJECT = 1, YL_OP_GRADIENT = 2, YL_OP_BACKWARD = 3, YL_OP_SLOTS = 4 };
 
 struct et_plan {
     int device = 0;
@@ -276,7 +281,9 @@ struct et_plan {
     // backprojection, exact attenuation gradient with attenuation): factors of
     // every view in view order, cached on the attenuation map's key
     bool yl_ops = false;
-    DevBuf fac, sino_t, op_tables, op_acc;  // (op_acc: backprojection accumulator, YF layout)
+    DevBuf fac, sino_t, op_acc, op_acc2;  // (op_acc, op_acc2: pull accumulators, YF layout)
+    DevBuf op_tables[YL_OP_SLOTS];        // per operation: its last table, and that table's bytes
+    std::vector<char> op_tables_h[YL_OP_SLOTS];
     bool fac_valid = false;
     unsigned long long fac_key = 0;
     // the last projection's partial sums [fwd_D][V][x'][rows] and the content
@@ -824,6 +831,8 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
                        int truncate);
 bool yl_op_gradient(et_plan *p, const float *sino, const float *act, const float *act_yf, const float *mu,
                     const float *mu_yf, float *out, int truncate);
+bool yl_op_backward(et_plan *p, const float *sino, const float *act, const float *act_yf, const float *mu,
+                    const float *mu_yf, float *out_act, float *out_mu, int truncate);
 
 void op_project(et_plan *p, const float *act, const float *mu, float *sino, int truncate)
 {
@@ -987,6 +996,22 @@ void op_gradient(et_plan *p, const float *sino, const float *act, const float *m
         if (!exact) conv_u_general(p, p->u.p, false);
         pull_general(p, v, p->u.p, exact, v > 0, truncate && v == p->V - 1, out, ET_EPI_ABI_STORE);
     }
+}
+
+// Both gradients of <sino, project(act, mu)>: the exact adjoint backprojection
+// of sino (activity) and the exact attenuation gradient (mu).  Row-lane plans
+// after a projection of the same pair take one pass (yl_op_backward); otherwise
+// the two operations in turn.
+void op_backward(et_plan *p, const float *sino, const float *act, const float *mu, float *out_act, float *out_mu,
+                 int truncate)
+{
+    if (p->fast && p->yl_ops && p->yl) {
+        pack(p, act, p->act_yf);
+        pack(p, mu, p->mu_yf);
+        if (yl_op_backward(p, sino, act, p->act_yf.p, mu, p->mu_yf.p, out_act, out_mu, truncate)) return;
+    }
+    op_backproject(p, sino, mu, out_act, true, truncate);
+    op_gradient(p, sino, act, mu, out_mu, true, truncate);
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1219,6 +1244,10 @@ EtSweepArgs yl_sweep_args(const et_plan *p, const float *recs, int nv, int D, co
     return a;
 }
 
+// Blocks of the row-lane pulls (reference and exact adjoint): 4 x 4 tiles of
+// image columns.
+int yl_pull_tiles(const et_plan *p) { return ((p->nx + 3) / 4) * ((p->nz + 3) / 4); }
+
 // Grouped launches of row-lane OSEM work over several plans; the block shapes
 // and templates come from each plan.  Sweeps share a launch when they share
 // (R, RPL, NR, block size); pulls when they share RPL.
@@ -1331,7 +1360,7 @@ struct YlTable {
                                                 (*sens_out)[i], nullptr, nullptr)
                                     : pull_args(p, r, nv[i], p->u.p, false, ET_EPI_OSEM, false, false, nullptr,
                                                 p->x_yf.p, sens[i]);
-            gx = (int)(((size_t)p->nx * p->nz + 15) / 16);
+            gx = yl_pull_tiles(p);
             gy = 1;
             return a;
         });
@@ -1363,9 +1392,11 @@ void yl_issue(const float *dev, const std::vector<YlLaunch> &L, size_t from, siz
             check_launch(et_launch_ratio_group(static_cast<const EtRatioArgs *>(args), tab, l.n, l.blocks, s));
             break;
         }
-        case YL_PULL_ADJ: {
+        case YL_PULL_ADJ:
+        case YL_PULL_ADJ2: {
             ProfScope ps(prof, ET_PROF_PULL, l.bytes);
-            check_launch(et_launch_yl_pull_adj(l.RPL, static_cast<const EtPullArgs *>(args), tab, l.n, l.blocks, s));
+            check_launch(et_launch_yl_pull_adj(l.RPL, l.kind == YL_PULL_ADJ2 ? 2 : 1,
+                                               static_cast<const EtPullArgs *>(args), tab, l.n, l.blocks, s));
             break;
         }
         default: {
@@ -1453,10 +1484,19 @@ float *yl_sino_t(et_plan *p, const float *sino)
     return t;
 }
 
-void yl_run(et_plan *p, const YlTable &t)
+// Issues a single operation's table from its slot; a table whose bytes equal
+// the slot's last upload (the same buffers and sizes: repeated calls) is on the
+// device already and is not uploaded again (no stream synchronisation).
+void yl_run(et_plan *p, const YlTable &t, int slot)
 {
-    yl_upload(p->op_tables, t, p->st);
-    yl_issue(p->op_tables.p, t.L, 0, t.L.size(), p->st, p);
+    DevBuf &buf = p->op_tables[slot];
+    std::vector<char> &last = p->op_tables_h[slot];
+    if (!buf.p || last != t.tb.bytes) {
+        last.clear();  // (an upload that throws leaves the slot unmatched)
+        yl_upload(buf, t, p->st);
+        last = t.tb.bytes;
+    }
+    yl_issue(buf.p, t.L, 0, t.L.size(), p->st, p);
 }
 
 // Adds to t the pull of nv views' rotated-frame volumes (p->u) into the YF
@@ -1468,7 +1508,7 @@ void yl_pull_into(YlTable &t, et_plan *p, const float *recs, int nv, bool adjoin
     t.grouped<EtPullArgs>(adjoint ? YL_PULL_ADJ : YL_PULL_STORE, {p}, YlTable::pull_key,
                           [&](int, int &gx, int &gy, double &bytes) {
                               bytes += (double)nv * 4.0 * p->nvox + (accumulate ? 8.0 : 4.0) * p->nvox;
-                              gx = (int)(((size_t)p->nx * p->nz + 15) / 16);
+                              gx = yl_pull_tiles(p);
                               gy = 1;
                               return pull_args(p, recs, nv, p->u.p, adjoint, ET_EPI_YF_STORE, accumulate, truncate,
                                                acc, nullptr, nullptr);
@@ -1497,7 +1537,7 @@ bool yl_op_project(et_plan *p, const float *act, const float *act_yf, const floa
         gy = p->V;
         return a;
     });
-    yl_run(p, t);
+    yl_run(p, t, YL_OP_PROJECT);
     ProfScope ps(p, ET_PROF_OTHER, (Deff + 1.0) * 4.0 * p->V * p->P);
     check_launch(
         et_launch_yl_parts_to_sino(part, Deff, p->V, p->nx, p->ny, p->yu, truncate, sino, (et_stream_t)p->st));
@@ -1539,7 +1579,7 @@ bool yl_op_backproject(et_plan *p, const float *sino, const float *mu, const flo
         });
         yl_pull_into(t, p, recs, nv, adjoint, v0 > 0, v0 + nv >= p->V && truncate, acc);
     }
-    yl_run(p, t);
+    yl_run(p, t, YL_OP_BACKPROJECT);
     ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
     check_launch(et_launch_unpack_yf(acc, out, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
     return true;
@@ -1603,9 +1643,74 @@ bool yl_op_gradient(et_plan *p, const float *sino, const float *act, const float
                                [&](int, int &gx, int &gy, double &bytes) { return args(gx, gy, bytes); });
         yl_pull_into(t, p, recs, nv, true, v0 > 0, v0 + nv >= p->V && truncate, acc);
     }
-    yl_run(p, t);
+    yl_run(p, t, YL_OP_GRADIENT);
     ProfScope ps(p, ET_PROF_OTHER, 8.0 * p->nvox);
     check_launch(et_launch_unpack_yf(acc, out, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
+    return true;
+}
+
+// Both gradients in one pass over the views, after a projection of the same
+// activity through the same map (the forward partial sums are reused, as
+// yl_op_gradient does): the adjoint backprojection sweep writes its rotated-frame
+// volumes into the first half of u, the gradient sweep into the second, and one
+// exact adjoint pull (YL_PULL_ADJ2), whose candidate weights serve both, adds
+// them up.  Each output equals its operation's on its own with all views in one
+// batch (the view batch set to the view count).  False (nothing issued) without
+// the forward sums or when the two volumes of every view would exceed 4 GiB.
+bool yl_op_backward(et_plan *p, const float *sino, const float *act, const float *act_yf, const float *mu,
+                    const float *mu_yf, float *out_act, float *out_mu, int truncate)
+{
+    if (!p->yl_ops || !p->yl || !act || !mu) return false;
+    const int V = p->V;
+    if (2.0 * V * p->ustride * sizeof(float) > 4.0 * (1ull << 30)) return false;
+    unsigned long long kmu, kact;
+    content_keys(p, mu, act, kmu, kact);
+    const int tiles = p->yl_tiles;
+    if (!(p->fwd_valid && p->fwd_act == kact && p->fwd_mu == kmu && p->fwd_tiles == tiles)) return false;
+    const float *fac = yl_view_factors(p, mu_yf, kmu);
+    const float *st = yl_sino_t(p, sino);
+    p->u.ensure((size_t)2 * V * p->ustride, true, p->st);
+    float *ub = p->u.p, *ug = p->u.p + (size_t)V * p->ustride;
+    float *acc_a = p->op_acc.ensure(p->yf_elems, true, p->st);
+    float *acc_m = p->op_acc2.ensure(p->yf_elems, true, p->st);
+    const float *recs = p->d_recs.p;
+    YlTable t;
+    const int Db = yl_op_chunks(p, V);
+    t.grouped<EtSweepArgs>(YL_BPU, {p}, YlTable::sweep_key, [&](int, int &gx, int &gy, double &bytes) {
+        bytes += (double)V * (8.0 * p->nvox + 4.0 * p->P);
+        EtSweepArgs a = yl_sweep_args(p, recs, V, Db, fac);
+        a.src = ET_SRC_SINO;
+        a.sino = st;
+        a.u = ub;
+        if (p->K > 0) a.psf = p->d_sep_flip.p;  // correlation: the adjoint of the PSF
+        gx = a.D;
+        gy = V;
+        return a;
+    });
+    t.grouped<EtSweepArgs>(YL_GRAD, {p}, YlTable::project_key, [&](int, int &gx, int &gy, double &bytes) {
+        bytes += (double)V * (8.0 * p->nvox + 4.0 * p->P);
+        EtSweepArgs a = yl_sweep_args(p, recs, V, p->fwd_D, fac);
+        a.act = const_cast<float *>(act_yf);
+        a.part = p->fwd_part.p;
+        a.sino = st;
+        a.u = ug;
+        gx = a.D * tiles;
+        gy = V;
+        return a;
+    });
+    t.grouped<EtPullArgs>(YL_PULL_ADJ2, {p}, YlTable::pull_key, [&](int, int &gx, int &gy, double &bytes) {
+        bytes += (double)V * 8.0 * p->nvox + 8.0 * p->nvox;
+        gx = yl_pull_tiles(p);
+        gy = 1;
+        EtPullArgs a = pull_args(p, recs, V, ub, true, ET_EPI_YF_STORE, false, truncate != 0, acc_a, nullptr, nullptr);
+        a.u2 = ug;
+        a.out2 = acc_m;
+        return a;
+    });
+    yl_run(p, t, YL_OP_BACKWARD);
+    ProfScope ps(p, ET_PROF_OTHER, 16.0 * p->nvox);
+    check_launch(et_launch_unpack_yf(acc_a, out_act, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
+    check_launch(et_launch_unpack_yf(acc_m, out_mu, p->nx, p->ny, p->nz, p->yp, p->gy, (et_stream_t)p->st));
     return true;
 }
 
@@ -3340,6 +3445,17 @@ int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *
 {
     if (!act || !mu) return niftyrec_error_parameters;
     ET_PLAN_CALL(plan, stream, op_gradient(plan, sino, act, mu, out, exact != 0, truncate))
+}
+
+int et_plan_project_backward(et_plan *plan, const float *sino, const float *act, const float *mu, float *out_act,
+                             float *out_mu, int truncate, void *stream)
+{
+    if (!sino || !act || !mu || !out_act || !out_mu) return niftyrec_error_parameters;
+    const void *in[3] = {sino, act, mu};
+    for (const void *q : in)
+        if (q == out_act || q == out_mu) return niftyrec_error_parameters;
+    if (out_act == out_mu) return niftyrec_error_parameters;
+    ET_PLAN_CALL(plan, stream, op_backward(plan, sino, act, mu, out_act, out_mu, truncate))
 }
 
 int et_plan_osem(et_plan *plan, float *act, const float *sino, const float *mu, int subsets, int iterations,

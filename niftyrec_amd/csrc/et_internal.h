@@ -101,6 +101,8 @@ struct EtPullArgs {
     int yp, gy;              // YF layout of out / x / sens
     float *x;                // ET_EPI_OSEM: activity (YF), updated in place
     const float *sens;       // ET_EPI_OSEM: clamped sensitivity (YF)
+    const float *u2;         // exact adjoint pull of two volumes of the same views
+    float *out2;             // (u into out, u2 into out2; null otherwise)
 };
 
 // MLEM ratio of one problem from its depth-chunk partial projections.
@@ -197,8 +199,10 @@ int et_launch_yl_pull(int epilogue, int RPL, const EtPullArgs *args, const void 
 // [v][z'][yu][roundup(nx, 64)] = e^{-A} from a.mu (YF); 16-row slabs.
 int et_launch_att_factors(const EtSweepArgs &a, et_stream_t stream);
 // Exact adjoint pull of the row-lane rotated-frame volumes into a.out (YF layout,
-// accumulate / truncate as ET_EPI_YF_STORE); grid as the row-lane pull.
-int et_launch_yl_pull_adj(int RPL, const EtPullArgs *args, const void *tab, int n, int blocks, et_stream_t stream);
+// accumulate / truncate as ET_EPI_YF_STORE); grid as the row-lane pull.  nsrc 2:
+// a.u into a.out and a.u2 into a.out2 with one set of candidate weights.
+int et_launch_yl_pull_adj(int RPL, int nsrc, const EtPullArgs *args, const void *tab, int n, int blocks,
+                          et_stream_t stream);
 // Attenuation factors of a.V views into a.att (row-lane layout) from a.mu (YF).
 int et_launch_yl_factors(int RPL, const EtSweepArgs &a, et_stream_t stream);
 // sino[v][y][x] = sum over d < D of part[d][v][x][y] (compensated), truncated

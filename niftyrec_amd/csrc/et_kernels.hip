@@ -1051,14 +1051,18 @@ int et_launch_att_factors(const EtSweepArgs &a, et_stream_t stream)
     return (int)hipGetLastError();
 }
 
-int et_launch_yl_pull_adj(int RPL, const EtPullArgs *args, const void *tab, int n, int blocks, et_stream_t stream)
+int et_launch_yl_pull_adj(int RPL, int nsrc, const EtPullArgs *args, const void *tab, int n, int blocks,
+                          et_stream_t stream)
 {
-    if (n <= 0 || blocks <= 0) return (int)hipErrorInvalidValue;
+    if (n <= 0 || blocks <= 0 || (nsrc != 1 && nsrc != 2)) return (int)hipErrorInvalidValue;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int4 *t = reinterpret_cast<const int4 *>(tab);
 #define ET_YLA(rpl)                                                                                  \
     if (RPL == rpl) {                                                                                \
-        hipLaunchKernelGGL((k_yl_pull_adj<rpl>), dim3(blocks), dim3(256), 0, st, args, t, n);        \
+        if (nsrc == 2)                                                                               \
+            hipLaunchKernelGGL((k_yl_pull_adj<rpl, 2>), dim3(blocks), dim3(256), 0, st, args, t, n); \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_yl_pull_adj<rpl, 1>), dim3(blocks), dim3(256), 0, st, args, t, n); \
         return (int)hipGetLastError();                                                               \
     }
     ET_YLA(2) ET_YLA(4) ET_YLA(6) ET_YLA(8)

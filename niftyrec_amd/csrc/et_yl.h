@@ -915,17 +915,18 @@ __global__ void __launch_bounds__(256) k_yl_factors(EtSweepArgs a)
 // reference inverse-rotation pull: negated angles in ZYX order, trilinear with
 // zero padding, views accumulated in order; emission-lib/_et.cpp:474-498), with
 // the OSEM update x *= max(b, 0) / sens (ET_EPI_OSEM) or the truncated store of
-// b (ET_EPI_YF_STORE) on the rows below ny.  Block: 16 image columns (x, z),
-// column blocks contiguous per XCD.  256 threads.
+// b (ET_EPI_YF_STORE) on the rows below ny.  Block: a 4 x 4 tile of image
+// columns (x, z), x fastest; tiles numbered x-major and contiguous per XCD.
+// 256 threads; grid ceil(nx/4) * ceil(nz/4).
 // ------------------------------------------------------------------------------
 template <int RPL, int EPI>
 __device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
 {
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int cb = ((b.gx & 7) == 0) ? (b.lin & 7) * (b.gx >> 3) + (b.lin >> 3) : b.lin;
-    const int c = cb * 16 + slot;
-    if (c >= a.nx * a.nz) return;
-    const int x = c % a.nx, z = c / a.nx;
+    const int ntx = (a.nx + 3) >> 2;
+    const int x = (cb % ntx) * 4 + (slot & 3), z = (cb / ntx) * 4 + (slot >> 2);
+    if (x >= a.nx || z >= a.nz) return;
     constexpr int rows = 16 * RPL;  // == a.yu
     const int row0 = RPL * lane;
     const int dx = rows, dz = rows * a.xp;
@@ -985,28 +986,38 @@ __device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
 // the row's non-zero ones are listed in LDS in candidate order (ballot, rank by
 // popcount), and the row takes them four at a time, the four loads in flight
 // together.  Stores as ET_EPI_YF_STORE (accumulate, truncate; rows below ny).
-// Block: 16 image columns, column blocks contiguous per XCD.  256 threads.
+// Block: a 4 x 4 tile of image columns (x fastest), as the reference pull's.
+// 256 threads; grid ceil(nx/4) * ceil(nz/4).
+// NS = 2: the same candidate lists pull a.u into a.out and a.u2 into a.out2,
+// two hits at a time (as many loads in flight as four hits of one volume);
+// each output sees the FMA sequence of a pull of its volume alone.
 // ------------------------------------------------------------------------------
-template <int RPL>
+template <int RPL, int NS>
 __device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk &b)
 {
+    static_assert(NS == 1 || NS == 2, "one or two volumes");
+    constexpr int HB = 4 / NS;  // hits per batch
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int cb = ((b.gx & 7) == 0) ? (b.lin & 7) * (b.gx >> 3) + (b.lin >> 3) : b.lin;
-    const int c = cb * 16 + slot;
-    if (c >= a.nx * a.nz) return;  // (the whole 16-lane row)
-    const int x = c % a.nx, z = c / a.nx;
+    const int ntx = (a.nx + 3) >> 2;
+    const int x = (cb % ntx) * 4 + (slot & 3), z = (cb / ntx) * 4 + (slot >> 2);
+    if (x >= a.nx || z >= a.nz) return;  // (the whole 16-lane row)
     constexpr int rows = 16 * RPL;  // == a.yu
     const int row0 = RPL * lane;
     const float fxo = (float)x, fzo = (float)z;
     __shared__ float hit_w[16][16];  // per row: the weights and offsets of its non-zero candidates
     __shared__ int hit_o[16][16];
     const int rsh = (int)threadIdx.x & 48;  // this row's first lane within the wave
-    float acc[RPL];
+    float acc[NS][RPL];
 #pragma unroll
-    for (int i = 0; i < RPL; i++) acc[i] = 0.f;
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int i = 0; i < RPL; i++) acc[s][i] = 0.f;
     for (int v = 0; v < a.V; v++) {
         const float *rec = a.views + (size_t)v * ET_VIEW_REC;
-        const float *uv = a.u + (size_t)v * a.ustride + row0;
+        const float *uv[NS];
+        uv[0] = a.u + (size_t)v * a.ustride + row0;
+        if constexpr (NS == 2) uv[1] = a.u2 + (size_t)v * a.ustride + row0;
         const float f00 = cload(rec + 0), f02 = cload(rec + 2), f03 = cload(rec + 3), f20 = cload(rec + 8),
                     f22 = cload(rec + 10), f23 = cload(rec + 11);
         const float i00 = cload(rec + 24), i02 = cload(rec + 26), i20 = cload(rec + 30), i22 = cload(rec + 32);
@@ -1040,37 +1051,45 @@ __device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk 
                 hit_o[slot][rank] = ok;
             }
             const int nh = __builtin_popcount(mask);  // the same for the whole row
-            for (int j = 0; j < nh; j += 4) {
-                float w[4], t[4][RPL];
+            for (int j = 0; j < nh; j += HB) {
+                float w[HB], t[HB][NS][RPL];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < HB; q++) {
                     const int h = j + q < nh ? j + q : j;  // past the list: reload hit j, weight unused
                     w[q] = hit_w[slot][h];
-                    yl_load<RPL>(uv + hit_o[slot][h], t[q]);
+                    const int o = hit_o[slot][h];
+#pragma unroll
+                    for (int s = 0; s < NS; s++) yl_load<RPL>(uv[s] + o, t[q][s]);
                 }
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < HB; q++) {
                     if (j + q < nh) {
 #pragma unroll
-                        for (int i = 0; i < RPL; i++) acc[i] = fmaf(t[q][i], w[q], acc[i]);
+                        for (int s = 0; s < NS; s++)
+#pragma unroll
+                            for (int i = 0; i < RPL; i++) acc[s][i] = fmaf(t[q][s][i], w[q], acc[s][i]);
                     }
                 }
             }
         }
     }
     const size_t base = (a.gy + row0) + (size_t)a.yp * ((x + 1) + (size_t)(a.nx + 2) * (z + 1));
-    float ov[RPL], o[RPL];
-    yl_load<RPL>(a.out + base, ov);
 #pragma unroll
-    for (int i = 0; i < RPL; i++) {
-        float val = a.accumulate ? ov[i] + acc[i] : acc[i];
-        if (a.truncate && val < 0.f) val = 0.f;
-        o[i] = (row0 + i < a.ny) ? val : ov[i];
+    for (int s = 0; s < NS; s++) {
+        float *out = s == 0 ? a.out : a.out2;
+        float ov[RPL], o[RPL];
+        yl_load<RPL>(out + base, ov);
+#pragma unroll
+        for (int i = 0; i < RPL; i++) {
+            float val = a.accumulate ? ov[i] + acc[s][i] : acc[s][i];
+            if (a.truncate && val < 0.f) val = 0.f;
+            o[i] = (row0 + i < a.ny) ? val : ov[i];
+        }
+        yl_store<RPL>(out + base, o);
     }
-    yl_store<RPL>(a.out + base, o);
 }
 
-template <int RPL>
+template <int RPL, int NS>
 __global__ void __launch_bounds__(256) k_yl_pull_adj(const EtPullArgs *__restrict__ args,
                                                      const int4 *__restrict__ tab, int n)
 {
@@ -1078,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_yl_pull_adj(const EtPullArgs *__restric
     int j;
     if (!group_block(tab, n, b, j)) return;
     const EtPullArgs a = ((ET_CONST_AS const EtPullArgs *)args)[j];
-    yl_pull_adj_body<RPL>(a, b);
+    yl_pull_adj_body<RPL, NS>(a, b);
 }
 
 template <int RPL, int EPI>

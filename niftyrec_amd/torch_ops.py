@@ -39,6 +39,10 @@ class _Project(torch.autograd.Function):
         g = grad_out.contiguous()
         plan = ctx.plan
         grad_act = grad_mu = None
+        if ctx.has_mu and ctx.needs_input_grad[0] and ctx.needs_input_grad[1]:
+            # both in one pass (after this projection, the plan reuses its depth-chunk sums)
+            grad_act, grad_mu = plan.project_backward(g, act, mu, truncate=False)
+            return grad_act, grad_mu, None
         if ctx.needs_input_grad[0]:
             grad_act = plan.backproject(g, mu, mode=ADJOINT, truncate=False)
         if ctx.has_mu and ctx.needs_input_grad[1]:

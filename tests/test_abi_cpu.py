@@ -117,6 +117,13 @@ def test_parameter_errors_before_device():
     plan = ctypes.c_void_p()
     assert A.et_plan_create(ctypes.byref(plan), 520, 2, 520, cams.ctypes.data_as(_lib.fp), 4, 1, None, 0,
                             ctypes.c_float(0), ctypes.c_float(0)) == 2
+    # the two-output backward: an output that aliases an input or the other output
+    # (the pointers are never dereferenced: the check comes first)
+    L = _lib.load()
+    s, a, m, o1, o2 = (ctypes.c_void_p(0x1000 * k) for k in range(1, 6))
+    for outs in ((s, o2), (o1, a), (m, o2), (o1, o1)):
+        assert L.et_plan_project_backward(ctypes.c_void_p(0x9000), s, a, m, outs[0], outs[1], 0, None) == 2
+    assert L.et_plan_project_backward(ctypes.c_void_p(0x9000), s, a, None, o1, o2, 0, None) == 2
 
 
 def test_no_device_is_a_status_not_an_exit():

@@ -180,3 +180,45 @@ def test_batch_of_sweep_kernel_plans():
         assert p.projector() == 0
         torch.cuda.synchronize()
         assert torch.equal(Xs, Xb[i]), "problem %d" % i
+
+
+@pytest.mark.parametrize("shape, V, K, truncate", [((40, 20, 40), 10, 5, False), ((96, 20, 96), 8, 7, True),
+                                                    ((130, 24, 130), 6, 11, False), ((17, 1, 17), 3, 3, False)])
+def test_project_backward_matches_the_two_operations(shape, V, K, truncate):
+    """et_plan_project_backward right after a projection of the same pair (one
+    pass: both rotated-frame sweeps, one exact adjoint pull for both outputs):
+    bit-identical to backproject(ADJOINT) and gradient_attenuation(exact) with
+    all views in one batch, and against the oracle.  Then with another
+    activity (no projection of it: the two operations in turn)."""
+    from niftyrec_amd.api import ADJOINT, Plan
+    act, mu, cams, psf = _case(shape, V, K, seed=101)
+    plan = Plan(act.shape, cams, psf)
+    plan.set_tuning(0, V)
+    A, M = _t(act), _t(mu)
+    s = np.random.default_rng(102).random((V,) + act.shape[1:], dtype=np.float32)
+    if truncate:
+        s -= np.float32(0.3)  # negative sums: the truncation matters
+    S = _t(s)
+    plan.project(A, M, truncate=False)
+    ga, gm = plan.project_backward(S, A, M, truncate=truncate)
+    ba = plan.backproject(S, M, mode=ADJOINT, truncate=truncate)
+    g = plan.gradient_attenuation(S, A, M, exact=True, truncate=truncate)
+    torch.cuda.synchronize()
+    assert torch.equal(ga, ba)
+    assert torch.equal(gm, g)
+    tag = "%dx%dx%d, %d views, K=%d%s" % (shape[2], shape[1], shape[0], V, K, ", truncated" if truncate else "")
+    tr = 1 if truncate else 0
+    report_parity("project_backward, activity (%s)" % tag, ga.cpu().numpy(),
+                  orc.backproject(s, mu, cams, psf, 0.0, 0.0, tr, mode=1),
+                  orc.backproject(s, mu, cams, psf, 0.0, 0.0, tr, mode=1, dtype=np.float64))
+    if not truncate:  # (the oracle's gradient is untruncated)
+        report_parity("project_backward, attenuation (%s)" % tag, gm.cpu().numpy(),
+                      orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float32),
+                      orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float64))
+    act2 = act * np.float32(0.5)
+    ga2, gm2 = plan.project_backward(S, _t(act2), M, truncate=truncate)
+    assert torch.equal(ga2, ba)  # (the activity's adjoint does not depend on the activity)
+    if not truncate:
+        report_parity("project_backward without the projection (%s)" % tag, gm2.cpu().numpy(),
+                      orc.gradient_mu(s, act2, mu, cams, psf, dtype=np.float32),
+                      orc.gradient_mu(s, act2, mu, cams, psf, dtype=np.float64))
