@@ -118,6 +118,20 @@ __device__ __forceinline__ float2 gload2_stream(const float *p)
     const et_f2v v = __builtin_nontemporal_load((g_cf2v *)p);
     return make_float2(v.x, v.y);
 }
+// Streaming (non-temporal) stores: data written once and not read again soon,
+// kept from displacing the caches' reused lines.
+typedef ET_GLOBAL_AS et_f4v g_f4v;
+typedef ET_GLOBAL_AS et_f2v g_f2v;
+__device__ __forceinline__ void gstore4_stream(float *p, float4 v)
+{
+    const et_f4v t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, (g_f4v *)p);
+}
+__device__ __forceinline__ void gstore2_stream(float *p, float2 v)
+{
+    const et_f2v t = {v.x, v.y};
+    __builtin_nontemporal_store(t, (g_f2v *)p);
+}
 // Read-only, block-uniform data (view records, PSF taps, argument records)
 // read through the constant address space: scalar loads.
 typedef ET_CONST_AS const float c_cfloat;

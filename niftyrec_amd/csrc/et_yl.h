@@ -96,6 +96,21 @@ __device__ __forceinline__ void yl_load_stream(const float *p, float *v)
     }
 }
 
+// yl_store through streaming stores (the gradient's rotated-frame volumes:
+// larger than the caches, read back by the pull only after the whole sweep).
+template <int RPL>
+__device__ __forceinline__ void yl_store_stream(float *p, const float *v)
+{
+    if constexpr (RPL % 4 == 0) {
+#pragma unroll
+        for (int q = 0; q < RPL / 4; q++)
+            gstore4_stream(p + 4 * q, make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]));
+    } else {
+#pragma unroll
+        for (int q = 0; q < RPL / 2; q++) gstore2_stream(p + 2 * q, make_float2(v[2 * q], v[2 * q + 1]));
+    }
+}
+
 template <int RPL>
 __device__ __forceinline__ void yl_store(float *p, const float *v)
 {
@@ -446,7 +461,7 @@ __device__ __forceinline__ void yl_grad_body(const EtSweepArgs &a, const Blk &b)
         yl_load<RPL>(a.sino + ((size_t)v * a.nx + xc) * rows + row0, sv);
 #pragma unroll
         for (int i = 0; i < RPL; i++) out[i] = -sv[i] * (acc[r][i] - cmp[r][i]);
-        yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), out);
+        yl_store_stream<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), out);
     };
 
     constexpr int PB = 16 / NR;  // planes per sample batch (descending)
@@ -731,7 +746,7 @@ __device__ __forceinline__ void yl_grad_tile_body(const EtSweepArgs &a, const Bl
                 yl_load<RPL>(a.sino + ((size_t)v * a.nx + xc) * rows + row0, sv);
 #pragma unroll
                 for (int i = 0; i < RPL; i++) out[i] = -sv[i] * (acc[r][i] - cmp[r][i]);
-                yl_store<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), out);
+                yl_store_stream<RPL>(uv + (size_t)rows * ((xc + 1) + (size_t)a.xp * (z + 1)), out);
             }
         }
     }
