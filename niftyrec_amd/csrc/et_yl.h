@@ -555,8 +555,8 @@ __device__ __forceinline__ void yl_project_tile_body(const EtSweepArgs &a, const
     const FastView fv = load_fast_view(a.views + (size_t)v * ET_VIEW_REC);
     yl_zero_lds(lds, 2 * PF);
     const float *act = a.act + a.gy + row0;
-    // column of this slot in round r: owned columns in rounds 0 and 1, the
-    // tile's halo columns in round 2 (slots 0 .. 2R - 1); -1 where none
+    // column of this slot in column pass r: owned columns in passes r = 0 and
+    // r = 1, the tile's halo columns in pass r = 2 (slots 0 .. 2R - 1); -1 where none
     auto col = [&](int r) -> int {
         if (r < NR) return x_lo + r * W + slot;
         if (slot < R) return x_lo - R + slot;

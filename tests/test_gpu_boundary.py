@@ -171,7 +171,7 @@ def spect64():
     return w, cams3, y
 
 
-def test_et_array_mlem_driver_config1(spect64):
+def test_et_array_mlem_driver_spect64(spect64):
     """et_array_mlem: 10 MLEM iterations of the 64^3 / 64-view / PSF 7 workload
     from a uniform start in one device-resident call, against 10 oracle
     updates (et_array_make_cameras' inclusive endpoints: last = 360 (1 - 1/64))."""

@@ -222,3 +222,29 @@ def test_project_backward_matches_the_two_operations(shape, V, K, truncate):
         report_parity("project_backward without the projection (%s)" % tag, gm2.cpu().numpy(),
                       orc.gradient_mu(s, act2, mu, cams, psf, dtype=np.float32),
                       orc.gradient_mu(s, act2, mu, cams, psf, dtype=np.float64))
+
+
+def test_project_backward_on_sweep_kernel_plans():
+    """A plan the row-lane kernels do not take (140 axial rows):
+    et_plan_project_backward runs the two operations in turn, bit-identical to
+    them, and torch autograd through it matches the oracle."""
+    from niftyrec_amd.api import ADJOINT, Plan
+    from niftyrec_amd.torch_ops import project
+    act, mu, cams, psf = _case((24, 140, 24), 4, 5, seed=103)
+    plan = Plan(act.shape, cams, psf)
+    A, M = _t(act), _t(mu)
+    s = np.random.default_rng(104).random((4,) + act.shape[1:], dtype=np.float32)
+    S = _t(s)
+    plan.project(A, M, truncate=False)
+    ga, gm = plan.project_backward(S, A, M)
+    assert plan.projector() == 0
+    assert torch.equal(ga, plan.backproject(S, M, mode=ADJOINT, truncate=False))
+    assert torch.equal(gm, plan.gradient_attenuation(S, A, M, exact=True, truncate=False))
+    Av, Mv = A.clone().requires_grad_(True), M.clone().requires_grad_(True)
+    (project(Av, Mv, plan) * S).sum().backward()
+    report_parity("autograd on the sweep kernels, d/d activity", Av.grad.cpu().numpy(),
+                  orc.backproject(s, mu, cams, psf, 0.0, 0.0, 0, mode=1),
+                  orc.backproject(s, mu, cams, psf, 0.0, 0.0, 0, mode=1, dtype=np.float64))
+    report_parity("autograd on the sweep kernels, d/d mu", Mv.grad.cpu().numpy(),
+                  orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float32),
+                  orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float64))
