@@ -295,8 +295,8 @@ struct et_plan {
     int fwd_D = 0, fwd_tiles = 0;  // (fwd_D: the depth chunks the projection asked yl_chunks for)
     std::vector<struct YlLaunch> yl_launches;          // per subset: projection, ratio, backprojection, pull
     std::vector<const void *> osem_sig;                // buffers yl_launches refer to
-    DevBuf fp_dev;                  // fingerprints (two of 8 bytes)
-    unsigned long long *fp_host = nullptr;  // pinned copies of the fingerprints
+    DevBuf fp_dev;                  // fingerprint partial sums (two arrays of ET_FP_BLOCKS)
+    unsigned long long *fp_host = nullptr;  // their pinned host copy
     // tuning
     int tune_D = 0, tune_vb = 0;
     // profiling
@@ -1190,27 +1190,46 @@ int tune_depth_chunks(et_plan *p, int mode, const float *recs, int nv, const flo
 // mu_key_read returns it once the stream has reached that point (or mu's
 // address when the plan's mu_check is off and `content` is not asked for, 0
 // without attenuation).
-void mu_key_launch(et_plan *p, const float *mu, bool content = false)
+// Queues the fingerprints of a and (when not null) b, n = nvox floats each,
+// and the copy of their partial sums to pinned host memory.
+void fp_launch(et_plan *p, const float *a, const float *b)
 {
-    if (!mu || !(p->mu_check || content)) return;
-    p->fp_dev.ensure(4, false);
-    if (!p->fp_host &&
-        hipHostMalloc((void **)&p->fp_host, 2 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess) {
+    p->fp_dev.ensure(2 * 2 * ET_FP_BLOCKS, false);  // (floats: two arrays of 8-byte partials)
+    if (!p->fp_host && hipHostMalloc((void **)&p->fp_host, 2 * ET_FP_BLOCKS * sizeof(unsigned long long),
+                                     hipHostMallocDefault) != hipSuccess) {
         p->fp_host = nullptr;
         throw EtError{niftyrec_error_alloccpu};
     }
-    ProfScope ps(p, ET_PROF_OTHER, 4.0 * p->nvox);
-    check_launch(et_launch_fingerprint(mu, p->nvox, reinterpret_cast<unsigned long long *>(p->fp_dev.p),
-                                       (et_stream_t)p->st));
-    check_hip(hipMemcpyAsync(p->fp_host, p->fp_dev.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->st),
+    const int nb = et_fingerprint_blocks(p->nvox);
+    unsigned long long *d = reinterpret_cast<unsigned long long *>(p->fp_dev.p);
+    ProfScope ps(p, ET_PROF_OTHER, (b ? 8.0 : 4.0) * p->nvox);
+    check_launch(et_launch_fingerprints(a, b, p->nvox, d, (et_stream_t)p->st));
+    check_hip(hipMemcpyAsync(p->fp_host, d, (b ? 2 : 1) * nb * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                             p->st),
               niftyrec_error_transfergpu);
+}
+
+// Key of array k (0: a, 1: b) of the last fp_launch, once the stream has
+// reached it: the sum modulo 2^64 of its partials.
+unsigned long long fp_key(const et_plan *p, int k)
+{
+    const int nb = et_fingerprint_blocks(p->nvox);
+    unsigned long long key = 0;
+    for (int i = 0; i < nb; i++) key += p->fp_host[(size_t)k * nb + i];
+    return key;
+}
+
+void mu_key_launch(et_plan *p, const float *mu, bool content = false)
+{
+    if (!mu || !(p->mu_check || content)) return;
+    fp_launch(p, mu, nullptr);
 }
 
 unsigned long long mu_key_read(const et_plan *p, const float *mu, bool content = false)
 {
     if (!mu) return 0;
     if (!(p->mu_check || content)) return (unsigned long long)(uintptr_t)mu;
-    return *p->fp_host;
+    return fp_key(p, 0);
 }
 
 
@@ -1431,17 +1450,12 @@ void yl_upload(DevBuf &buf, const YlTable &t, hipStream_t st)
 // see other maps between OSEM calls, and a map rewritten in place is caught.
 void content_keys(et_plan *p, const float *mu, const float *act, unsigned long long &kmu, unsigned long long &kact)
 {
-    mu_key_launch(p, mu, true);
-    if (act) {
-        ProfScope ps(p, ET_PROF_OTHER, 4.0 * p->nvox);
-        unsigned long long *d = reinterpret_cast<unsigned long long *>(p->fp_dev.p) + 1;
-        check_launch(et_launch_fingerprint(act, p->nvox, d, (et_stream_t)p->st));
-        check_hip(hipMemcpyAsync(p->fp_host + 1, d, sizeof(unsigned long long), hipMemcpyDeviceToHost, p->st),
-                  niftyrec_error_transfergpu);
-    }
+    kmu = kact = 0;
+    if (!mu && !act) return;
+    fp_launch(p, mu ? mu : act, mu ? act : nullptr);  // one launch for both
     check_hip(hipStreamSynchronize(p->st), niftyrec_error_kernel);
-    kmu = mu_key_read(p, mu, true);
-    kact = act ? p->fp_host[1] : 0;
+    if (mu) kmu = fp_key(p, 0);
+    if (act) kact = fp_key(p, mu ? 1 : 0);
 }
 
 // Factors of every view in view order ([v][z'][x'][rows]) for the map mu (mu_yf

@@ -172,9 +172,15 @@ int et_launch_ratio_group(const EtRatioArgs *args, const void *tab, int n, int b
 // what lies beyond plane z, sino = the whole ray; act or mu may be null.
 int et_launch_partial_line(const float *act, const float *mu, const float *bgimg, float bg, int nx, int ny, int nz,
                            float *sino, float *part, et_stream_t stream);
-// Content fingerprint of n floats into *out (8 bytes of device memory): the
-// key under which the OSEM caches remember the attenuation map.
-int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_stream_t stream);
+// Content fingerprints of p0 and (when not null) p1, n floats each: the keys
+// under which the caches remember the attenuation map and the activity.  The
+// launch stores et_fingerprint_blocks(n) partial sums per array in parts (p0's,
+// then p1's; at most 2 * ET_FP_BLOCKS); a key is the sum modulo 2^64 of its
+// array's partials.
+#define ET_FP_BLOCKS 256
+int et_fingerprint_blocks(size_t n);
+int et_launch_fingerprints(const float *p0, const float *p1, size_t n, unsigned long long *parts,
+                           et_stream_t stream);
 // Row-lane kernels of the OSEM subset loop (et_yl.h).  Grouped launches only
 // (tab as et_launch_sweep_group): sweeps kind 0 projection, 1 rotated-frame
 // backprojection, 3 exact attenuation gradient (reads the projection's partial

@@ -620,9 +620,8 @@ __global__ void __launch_bounds__(256) k_ratio_parts(const float *__restrict__ m
 }
 
 // Content fingerprint of n floats: the sum modulo 2^64 of a 64-bit mix of
-// (bits, index) per element (order-independent, so the block partials are
-// combined with one atomic add each).  Any change of one element changes the
-// sum.  out must hold zero before the launch.
+// (bits, index) per element (order-independent: any partition into blocks
+// gives the same sum).  Any change of one element changes the sum.
 __device__ __forceinline__ unsigned long long fp_mix(unsigned long long h)
 {
     h ^= h >> 33;
@@ -633,16 +632,19 @@ __device__ __forceinline__ unsigned long long fp_mix(unsigned long long h)
     return h;
 }
 
-// The key is a sum modulo 2^64 of per-element hashes: any grid and order give
-// the same value.  Few blocks (one atomic each) and 16-byte loads where p is
-// 16-byte aligned (vec4).
+// Fingerprints of one or two arrays of n floats (p0, p1): blocks [0, nb) hash
+// p0, blocks [nb, 2 nb) p1, and block b stores its partial sum in parts[b] (no
+// atomics, nothing to clear first); an array's key is the sum modulo 2^64 of
+// its nb partials.  16-byte loads where both arrays are 16-byte aligned (VEC4).
 template <bool VEC4>
-__global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict__ p, size_t n,
-                                                     unsigned long long *__restrict__ out)
+__global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict__ p0, const unsigned *__restrict__ p1,
+                                                     size_t n, int nb, unsigned long long *__restrict__ parts)
 {
     constexpr unsigned long long G = 0x9e3779b97f4a7c15ull;
+    const int arr = (int)blockIdx.x / nb, lb = (int)blockIdx.x - arr * nb;
+    const unsigned *__restrict__ p = arr ? p1 : p0;
     unsigned long long h = 0;
-    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    const size_t t0 = (size_t)lb * blockDim.x + threadIdx.x, stride = (size_t)nb * blockDim.x;
     size_t done = 0;
     if constexpr (VEC4) {
         const size_t n4 = n / 4;
@@ -662,7 +664,7 @@ __global__ void __launch_bounds__(256) k_fingerprint(const unsigned *__restrict_
     __shared__ unsigned long long part[4];
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = h;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+    if (threadIdx.x == 0) parts[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
 }
 
 // dst[v][x][y] = src[order[v]][y][x] for y < ny, 0 for ny <= y < rows: the
@@ -992,16 +994,20 @@ int et_launch_ratio_parts(const float *meas, const float *part, int D, size_t VP
     return (int)hipGetLastError();
 }
 
-int et_launch_fingerprint(const float *p, size_t n, unsigned long long *out, et_stream_t stream)
+int et_fingerprint_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(nblk(n, 1024), ET_FP_BLOCKS)); }
+
+int et_launch_fingerprints(const float *p0, const float *p1, size_t n, unsigned long long *parts, et_stream_t stream)
 {
+    if (!p0) return (int)hipErrorInvalidValue;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(k_fill, dim3(1), dim3(256), 0, st, reinterpret_cast<float *>(out), (size_t)2, 0.f);
-    const unsigned blocks = std::max(1u, (unsigned)std::min<size_t>(nblk(n, 1024), 512));
-    const unsigned *q = reinterpret_cast<const unsigned *>(p);
-    if ((reinterpret_cast<uintptr_t>(p) & 15) == 0)
-        hipLaunchKernelGGL(k_fingerprint<true>, dim3(blocks), dim3(256), 0, st, q, n, out);
+    const int nb = et_fingerprint_blocks(n);
+    const unsigned *q0 = reinterpret_cast<const unsigned *>(p0), *q1 = reinterpret_cast<const unsigned *>(p1);
+    const bool vec4 = (reinterpret_cast<uintptr_t>(p0) & 15) == 0 && (reinterpret_cast<uintptr_t>(p1) & 15) == 0;
+    const dim3 grid(p1 ? 2 * nb : nb);
+    if (vec4)
+        hipLaunchKernelGGL(k_fingerprint<true>, grid, dim3(256), 0, st, q0, q1, n, nb, parts);
     else
-        hipLaunchKernelGGL(k_fingerprint<false>, dim3(blocks), dim3(256), 0, st, q, n, out);
+        hipLaunchKernelGGL(k_fingerprint<false>, grid, dim3(256), 0, st, q0, q1, n, nb, parts);
     return (int)hipGetLastError();
 }
 
