@@ -1498,17 +1498,26 @@ float *yl_sino_t(et_plan *p, const float *sino)
     return t;
 }
 
-// Issues a single operation's table from its slot; a table whose bytes equal
-// the slot's last upload (the same buffers and sizes: repeated calls) is on the
-// device already and is not uploaded again (no stream synchronisation).
+// Issues a single operation's table from its slot.  A table equal to the
+// slot's last upload (the same launches over the same buffers and sizes:
+// repeated calls) is on the device already and is not uploaded again (no
+// stream synchronisation); the comparison covers the device bytes and each
+// launch's kernel choice and offsets.
 void yl_run(et_plan *p, const YlTable &t, int slot)
 {
+    std::vector<char> sig = t.tb.bytes;
+    for (const YlLaunch &l : t.L) {
+        const long long f[] = {l.kind, l.R, l.RPL, l.NR, l.nthr, (long long)l.lds, (long long)l.args_off,
+                               (long long)l.tab_off, l.n, l.blocks};
+        const char *c = reinterpret_cast<const char *>(f);
+        sig.insert(sig.end(), c, c + sizeof(f));
+    }
     DevBuf &buf = p->op_tables[slot];
     std::vector<char> &last = p->op_tables_h[slot];
-    if (!buf.p || last != t.tb.bytes) {
+    if (!buf.p || last != sig) {
         last.clear();  // (an upload that throws leaves the slot unmatched)
         yl_upload(buf, t, p->st);
-        last = t.tb.bytes;
+        last.swap(sig);
     }
     yl_issue(buf.p, t.L, 0, t.L.size(), p->st, p);
 }

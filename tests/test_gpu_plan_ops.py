@@ -248,3 +248,25 @@ def test_project_backward_on_sweep_kernel_plans():
     report_parity("autograd on the sweep kernels, d/d mu", Mv.grad.cpu().numpy(),
                   orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float32),
                   orc.gradient_mu(s, act, mu, cams, psf, dtype=np.float64))
+
+
+def test_repeated_operations_with_new_content():
+    """Repeated single operations over the plan's same work buffers reuse the
+    launch table on the device (not uploaded again); each call still sees its
+    own inputs: two activities, then two sinograms, each against the oracle."""
+    from niftyrec_amd.api import ADJOINT, Plan
+    act, mu, cams, psf = _case((48, 20, 48), 6, 5, seed=105)
+    plan = Plan(act.shape, cams, psf)
+    M = _t(mu)
+    rng = np.random.default_rng(106)
+    for k in range(2):
+        a = act * np.float32(1.0 + k) + rng.random(act.shape, dtype=np.float32) * np.float32(0.1)
+        p = plan.project(_t(a), M).cpu().numpy()
+        report_parity("repeated plan project, call %d" % k, p, orc.project(a, mu, cams, psf),
+                      orc.project(a, mu, cams, psf, dtype=np.float64))
+    for k in range(2):
+        y = rng.random((6,) + act.shape[1:], dtype=np.float32)
+        b = plan.backproject(_t(y), M, mode=ADJOINT, truncate=False).cpu().numpy()
+        report_parity("repeated plan backproject, call %d" % k, b,
+                      orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1),
+                      orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1, dtype=np.float64))
