@@ -177,7 +177,7 @@ int et_launch_partial_line(const float *act, const float *mu, const float *bgimg
 // launch stores et_fingerprint_blocks(n) partial sums per array in parts (p0's,
 // then p1's; at most 2 * ET_FP_BLOCKS); a key is the sum modulo 2^64 of its
 // array's partials.
-#define ET_FP_BLOCKS 256
+#define ET_FP_BLOCKS 1024
 int et_fingerprint_blocks(size_t n);
 int et_launch_fingerprints(const float *p0, const float *p1, size_t n, unsigned long long *parts,
                            et_stream_t stream);
