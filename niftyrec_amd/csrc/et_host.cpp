@@ -1264,8 +1264,8 @@ EtSweepArgs yl_sweep_args(const et_plan *p, const float *recs, int nv, int D, co
 }
 
 // Blocks of the row-lane pulls (reference and exact adjoint): 4 x 4 tiles of
-// image columns.
-int yl_pull_tiles(const et_plan *p) { return ((p->nx + 3) / 4) * ((p->nz + 3) / 4); }
+// image columns in super-tiles of 8 x 8 tiles.
+int yl_pull_tiles(const et_plan *p) { return (((p->nx + 3) / 4 + 7) / 8) * (((p->nz + 3) / 4 + 7) / 8) * 64; }
 
 // Grouped launches of row-lane OSEM work over several plans; the block shapes
 // and templates come from each plan.  Sweeps share a launch when they share

@@ -931,16 +931,19 @@ __global__ void __launch_bounds__(256) k_yl_factors(EtSweepArgs a)
 // zero padding, views accumulated in order; emission-lib/_et.cpp:474-498), with
 // the OSEM update x *= max(b, 0) / sens (ET_EPI_OSEM) or the truncated store of
 // b (ET_EPI_YF_STORE) on the rows below ny.  Block: a 4 x 4 tile of image
-// columns (x, z), x fastest; tiles numbered x-major and contiguous per XCD.
-// 256 threads; grid ceil(nx/4) * ceil(nz/4).
+// columns (x, z), x fastest; tiles numbered in 8 x 8-tile super-tiles (x-major
+// within and between them), contiguous per XCD, so that an XCD covers a
+// compact image region (32 x 32 columns per super-tile).  256 threads; grid
+// 64 * ceil(nx/32) * ceil(nz/32), blocks past the image exit at once.
 // ------------------------------------------------------------------------------
 template <int RPL, int EPI>
 __device__ __forceinline__ void yl_pull_body(const EtPullArgs &a, const Blk &b)
 {
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int cb = ((b.gx & 7) == 0) ? (b.lin & 7) * (b.gx >> 3) + (b.lin >> 3) : b.lin;
-    const int ntx = (a.nx + 3) >> 2;
-    const int x = (cb % ntx) * 4 + (slot & 3), z = (cb / ntx) * 4 + (slot >> 2);
+    const int ntx = (a.nx + 3) >> 2, nsx = (ntx + 7) >> 3;
+    const int sup = cb >> 6, t = cb & 63;
+    const int x = ((sup % nsx) * 8 + (t & 7)) * 4 + (slot & 3), z = ((sup / nsx) * 8 + (t >> 3)) * 4 + (slot >> 2);
     if (x >= a.nx || z >= a.nz) return;
     constexpr int rows = 16 * RPL;  // == a.yu
     const int row0 = RPL * lane;
@@ -1014,8 +1017,9 @@ __device__ __forceinline__ void yl_pull_adj_body(const EtPullArgs &a, const Blk 
     constexpr int HB = 4 / NS;  // hits per batch
     const int lane = threadIdx.x & 15, slot = threadIdx.x >> 4;
     const int cb = ((b.gx & 7) == 0) ? (b.lin & 7) * (b.gx >> 3) + (b.lin >> 3) : b.lin;
-    const int ntx = (a.nx + 3) >> 2;
-    const int x = (cb % ntx) * 4 + (slot & 3), z = (cb / ntx) * 4 + (slot >> 2);
+    const int ntx = (a.nx + 3) >> 2, nsx = (ntx + 7) >> 3;
+    const int sup = cb >> 6, t = cb & 63;
+    const int x = ((sup % nsx) * 8 + (t & 7)) * 4 + (slot & 3), z = ((sup / nsx) * 8 + (t >> 3)) * 4 + (slot >> 2);
     if (x >= a.nx || z >= a.nz) return;  // (the whole 16-lane row)
     constexpr int rows = 16 * RPL;  // == a.yu
     const int row0 = RPL * lane;
