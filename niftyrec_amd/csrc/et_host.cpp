@@ -476,11 +476,16 @@ hipEvent_t pool_event(et_plan *p)
 
 // Times the launches issued during its lifetime when category `cat` is
 // selected (et_plan_profile_select); the events come from the plan's pool.
+// Profiling scope: events recorded on the stream before and after the scope's
+// launch; for a projection, a second pair is offered to the launch, and a
+// row-lane projector takes it and records its own dispatch start and end (the
+// kernel's duration without the stream's gaps), which is then used instead.
 struct ProfScope {
     et_plan *p;
     int cat;
     double bytes;
     hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t ka = nullptr, kb = nullptr;  // offered to the launch
     ProfScope(et_plan *p_, int cat_, double bytes_) : p(p_), cat(cat_), bytes(bytes_)
     {
         if (!p || !((p->prof >> cat) & 1u)) return;
@@ -493,11 +498,33 @@ struct ProfScope {
             a = b = nullptr;
             return;
         }
+        if (cat == ET_PROF_PROJECT) {
+            ka = pool_event(p);
+            kb = pool_event(p);
+            if (ka && kb) {
+                et_offer_launch_events(ka, kb);
+            } else {
+                if (ka) p->ev_pool.push_back(ka);
+                if (kb) p->ev_pool.push_back(kb);
+                ka = kb = nullptr;
+            }
+        }
         (void)hipEventRecord(a, p->st);
     }
     ~ProfScope()
     {
         if (!a) return;
+        const bool kernel_timed = ka && et_withdraw_launch_events();
+        if (kernel_timed) {
+            p->events.push_back({cat, bytes, ka, kb});
+            p->ev_pool.push_back(a);
+            p->ev_pool.push_back(b);
+            return;
+        }
+        if (ka) {
+            p->ev_pool.push_back(ka);
+            p->ev_pool.push_back(kb);
+        }
         (void)hipEventRecord(b, p->st);
         p->events.push_back({cat, bytes, a, b});
     }

@@ -118,6 +118,16 @@ struct EtRatioArgs {
 // ---- launchers (et_kernels.hip) -------------------------------------------------
 // All return hipSuccess or the launch error; all are asynchronous on `stream`.
 typedef struct ihipStream_t *et_stream_t;
+typedef struct ihipEvent_t *et_event_t;
+
+// Kernel-timed projector launches: a profiling scope offers an event pair for
+// the next row-lane projector launch of this thread (et_offer_launch_events);
+// that launch records its own dispatch start and end in them
+// (hipExtLaunchKernelGGL) and takes them (et_claim_launch_events).
+// et_withdraw_launch_events clears the offer and returns whether a launch took it.
+void et_offer_launch_events(et_event_t start, et_event_t stop);
+bool et_claim_launch_events(et_event_t *start, et_event_t *stop);
+bool et_withdraw_launch_events();
 
 // has_mu: 0 no attenuation; 1 attenuation from the resampled a.mu; 2 (PROJECT)
 // as 1, and the attenuation factors stored in a.att; 3 (PROJECT, BPU) the

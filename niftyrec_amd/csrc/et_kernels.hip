@@ -994,6 +994,33 @@ int et_launch_ratio_parts(const float *meas, const float *part, int D, size_t VP
     return (int)hipGetLastError();
 }
 
+namespace {
+struct LaunchEvents {
+    et_event_t start = nullptr, stop = nullptr;
+    bool taken = false;
+};
+thread_local LaunchEvents g_launch_events;
+}  // namespace
+
+void et_offer_launch_events(et_event_t start, et_event_t stop) { g_launch_events = {start, stop, false}; }
+
+bool et_claim_launch_events(et_event_t *start, et_event_t *stop)
+{
+    LaunchEvents &e = g_launch_events;
+    if (!e.start || !e.stop || e.taken) return false;
+    *start = e.start;
+    *stop = e.stop;
+    e.taken = true;
+    return true;
+}
+
+bool et_withdraw_launch_events()
+{
+    const bool taken = g_launch_events.taken;
+    g_launch_events = LaunchEvents{};
+    return taken;
+}
+
 int et_fingerprint_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>(nblk(n, 1024), ET_FP_BLOCKS)); }
 
 int et_launch_fingerprints(const float *p0, const float *p1, size_t n, unsigned long long *parts, et_stream_t stream)

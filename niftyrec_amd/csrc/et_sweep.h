@@ -4,6 +4,8 @@
 // is instantiated in its own translation unit (et_sweep_r.hip, compiled once
 // per radius with -DET_SWEEP_R=R) so that the library builds in parallel.
 #pragma once
+#include <hip/hip_ext.h>
+
 #include "et_device.h"
 #include "et_yl.h"
 
@@ -598,6 +600,17 @@ static int launch_sweep_r(int mode, int has_mu, const EtSweepArgs &a, hipStream_
 // backprojection, 2 tiled projection, 3 exact attenuation gradient, 4 its tiled
 // variant; a grouped launch of `blocks` blocks of nthr = 16 * W threads
 // with `lds` bytes of dynamic LDS (two planes of the widest problem).
+// A row-lane projector launch, timed by its own dispatch (start and end
+// recorded in the events a profiling scope offered) where there is an offer.
+#define ET_YL_PROJECTOR_LAUNCH(kern)                                                                           \
+    {                                                                                                          \
+        et_event_t ea, eb;                                                                                     \
+        if (et_claim_launch_events(&ea, &eb))                                                                  \
+            hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(nthr), (uint32_t)lds, st, ea, eb, 0u, args, tab, n); \
+        else                                                                                                   \
+            hipLaunchKernelGGL(kern, dim3(blocks), dim3(nthr), lds, st, args, tab, n);                         \
+    }
+
 template <int R>
 static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const EtSweepArgs *args, const int4 *tab,
                        int n, int blocks, hipStream_t st)
@@ -607,7 +620,7 @@ static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const Et
         if (RPL == rpl) {                                                                                         \
             if (kind == 2) {                                                                                      \
                 allow_lds(k_yl_project_tile<R, rpl>, lds);                                                        \
-                hipLaunchKernelGGL((k_yl_project_tile<R, rpl>), dim3(blocks), dim3(nthr), lds, st, args, tab, n); \
+                ET_YL_PROJECTOR_LAUNCH((k_yl_project_tile<R, rpl>))                                               \
             } else {                                                                                              \
                 allow_lds(k_yl_grad_tile<R, rpl>, lds);                                                           \
                 hipLaunchKernelGGL((k_yl_grad_tile<R, rpl>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);    \
@@ -624,7 +637,7 @@ static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const Et
     if (RPL == rpl && NR == nr) {                                                                                 \
         if (kind == 0) {                                                                                          \
             allow_lds(k_yl_project<R, rpl, nr>, lds);                                                             \
-            hipLaunchKernelGGL((k_yl_project<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);      \
+            ET_YL_PROJECTOR_LAUNCH((k_yl_project<R, rpl, nr>))                                                    \
         } else if (kind == 3) {                                                                                   \
             allow_lds(k_yl_grad<R, rpl, nr>, lds);                                                                \
             hipLaunchKernelGGL((k_yl_grad<R, rpl, nr>), dim3(blocks), dim3(nthr), lds, st, args, tab, n);         \
@@ -639,6 +652,7 @@ static int launch_yl_r(int kind, int RPL, int NR, int nthr, size_t lds, const Et
 #undef ET_YL
     return (int)hipErrorInvalidValue;
 }
+#undef ET_YL_PROJECTOR_LAUNCH
 
 // Per-radius launchers (et_sweep_r.hip): sweep of `mode` with PSF radius R,
 // one problem (grp == nullptr) or a grouped launch; row-lane sweeps.
