@@ -270,3 +270,23 @@ def test_repeated_operations_with_new_content():
         report_parity("repeated plan backproject, call %d" % k, b,
                       orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1),
                       orc.backproject(y, mu, cams, psf, 0.0, 0.0, 0, mode=1, dtype=np.float64))
+
+
+def test_profile_of_sweep_kernel_plans():
+    """Profiling a plan the row-lane kernels do not take: its projections do
+    not take the events offered to a row-lane projector launch and are timed
+    by events recorded around them; every category holds its launches."""
+    from niftyrec_amd.api import Plan
+    act, mu, cams, psf = _case((24, 140, 24), 8, 5, seed=107)
+    plan = Plan(act.shape, cams, psf)
+    plan.set_tuning(4, 0)
+    Y, M = _t(orc.project(act, mu, cams, psf)), _t(mu)
+    X = torch.ones(act.shape, dtype=torch.float32, device="cuda")
+    plan.osem(X, Y, M, subsets=2, iterations=1)  # caches built outside the count
+    plan.profile(True)
+    plan.osem(X, Y, M, subsets=2, iterations=2)
+    r = plan.profile_read()
+    plan.profile(False)
+    assert plan.projector() == 0
+    for cat in ("project", "bp_rotated_frame", "bp_pull"):
+        assert r[cat][0] >= 4 and r[cat][1] > 0, (cat, r[cat])  # (at least one launch per subset update)
