@@ -117,6 +117,14 @@ def test_parameter_errors_before_device():
     plan = ctypes.c_void_p()
     assert A.et_plan_create(ctypes.byref(plan), 520, 2, 520, cams.ctypes.data_as(_lib.fp), 4, 1, None, 0,
                             ctypes.c_float(0), ctypes.c_float(0)) == 2
+    # empty inputs: no cameras, a volume with no planes, plans of no views or no columns
+    cams0 = np.zeros(0, np.float32)
+    assert refabi.project(A, act, None, cams0)[0] == 2
+    assert refabi.project(A, np.ones((0, 6, 8), np.float32), None, cams)[0] == 2
+    assert A.et_plan_create(ctypes.byref(plan), 8, 6, 8, cams0.ctypes.data_as(_lib.fp), 0, 1, None, 0,
+                            ctypes.c_float(0), ctypes.c_float(0)) == 2
+    assert A.et_plan_create(ctypes.byref(plan), 0, 6, 8, cams.ctypes.data_as(_lib.fp), 4, 1, None, 0,
+                            ctypes.c_float(0), ctypes.c_float(0)) == 2
     # the two-output backward: an output that aliases an input or the other output
     # (the pointers are never dereferenced: the check comes first)
     L = _lib.load()
