@@ -21,6 +21,22 @@ def L():
     return _lib.load()
 
 
+def test_header_is_plain_c(tmp_path):
+    """The boundary header compiles as strict C99 and as C++ (the bindings a
+    caller writes: ctypes, cgo, MATLAB mex, C++)."""
+    import shutil
+    import subprocess
+    src = tmp_path / "h.c"
+    src.write_text('#include "et_array_interface.h"\nint main(void) { return 0; }\n')
+    for cc, flags in (("gcc", ["-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror"]),
+                      ("g++", ["-std=c++17", "-Wall", "-Werror", "-x", "c++"])):
+        if shutil.which(cc) is None:
+            pytest.skip(cc + " not found")
+        r = subprocess.run([cc] + flags + ["-I", os.path.dirname(HEADER), "-c", str(src), "-o",
+                                          str(tmp_path / "h.o")], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
 def _declared():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
