@@ -279,8 +279,10 @@ int et_plan_gradient_attenuation(et_plan *plan, const float *sino, const float *
  * et_plan_project of the same act and mu, plans on the row-lane kernels run
  * both rotated-frame sweeps and one exact adjoint pull whose candidate weights
  * serve both outputs (rotated-frame volumes of all views twice: at most
- * 4 GiB); otherwise the two operations in turn.  Outputs must not alias each
- * other or an input (niftyrec_error_parameters). */
+ * 4 GiB); otherwise the two operations in turn.  Outputs must not overlap
+ * each other or an input: an output pointer equal to an input's or to the
+ * other output's is rejected (niftyrec_error_parameters); partial overlaps
+ * are not detected. */
 int et_plan_project_backward(et_plan *plan, const float *sino, const float *act,
                              const float *mu, float *out_act, float *out_mu, int truncate,
                              void *stream);
